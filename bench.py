@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""bench.py -- voxel-updates/s of the 8-field cytokine-diffusion step.
+
+Workload (default): 8 FP32 fields on a 1024x512x128 patch grid (x fastest),
+zero-padded linear convolution on the 2048x1024x256 FFT box, per-field
+normalized Gaussian Green's kernel (sigma^2 = 2 D dt, D ~ U[0.5,5], dt = 1)
+times exp(-lambda dt) (lambda ~ U[0.001,0.05]); lognormal(-2,1) background
+plus 50 point sources of amplitude U[1,100].  Inputs are synthetic and
+generated on the device; kernel spectra are precomputed and resident.
+
+A step = one tick of all fields (5 HBM passes per field, all fields batched
+into each launch).  W untimed warmup steps, then K timed steps bracketed by
+torch.cuda.synchronize().  Per-pass durations come from HIP events recorded
+between the launches on the launch stream over the timed region.  The CPU
+baseline is the oracle's FP32 OpenMP port timed on a bounded sample (one field
+of the same workload, one tick) on the host cores.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vf-version_7_3_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--shape", default="128,512,1024", help="nz,ny,nx of the data box")
+    ap.add_argument("--fields", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def synth_fields(shape, nfields, seed, device):
+    import numpy as np
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    n = shape[0] * shape[1] * shape[2]
+    fields, params = [], []
+    for _ in range(nfields):
+        f = torch.randn(shape, generator=g, device=device, dtype=torch.float32)
+        f.mul_(1.0).add_(-2.0).exp_()
+        idx = torch.randint(0, n, (50,), generator=g, device=device)
+        amp = torch.rand(50, generator=g, device=device) * 99.0 + 1.0
+        f.view(-1).index_add_(0, idx, amp)
+        fields.append(f)
+        D = rng.uniform(0.5, 5.0)
+        lam = rng.uniform(0.001, 0.05)
+        params.append((2.0 * D * 1.0, float(np.exp(-lam * 1.0))))
+    return fields, params
+
+
+def cpu_baseline(shape, seed):
+    """Oracle FP32 OpenMP port on one field of the same workload, one tick."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import orc
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    u = orc.synth_field(shape, seed, nsrc=50)
+    box = tuple(2 * s for s in shape)
+    ks = orc.gauss_spectrum_real(2.0 * 2.75, float(np.exp(-0.0255)), box, 1.0 / float(np.prod(box)))
+    t0 = time.perf_counter()
+    orc.cpu_diffuse_f32(u, ks, nthreads)
+    dt = time.perf_counter() - t0
+    nvox = float(np.prod(shape))
+    return {"value": nvox / dt, "unit": "voxel-updates/s", "cores": nthreads, "kind": "port",
+            "sample": f"1 field x 1 tick of the {shape[2]}x{shape[1]}x{shape[0]} workload "
+                      f"({dt:.2f} s, oracle FP32 OpenMP port, same pruned 5-pass algorithm)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(fields shard across GPUs with no exchange; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import vfd
+
+    shape = tuple(int(s) for s in args.shape.split(","))
+    nz, ny, nx = shape
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    plan = vfd.Plan(shape)
+    fields, params = synth_fields(shape, args.fields, args.seed, dev)
+    works = [plan.new_work(dev) for _ in fields]
+    kspecs = [plan.gauss_spectrum(s2, dec, stream=stream) for (s2, dec) in params]
+    arr = plan.field_array([{"in": f, "out": f, "work": w, "kspec": k}
+                            for f, w, k in zip(fields, works, kspecs)])
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        plan.diffuse(arr, stream)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        evs[k][0].record(stream)
+        for p in range(5):
+            plan.diffuse_pass(arr, p, stream)
+            evs[k][p + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    pass_ms = [0.0] * 5
+    step_ms = []
+    for k in range(K):
+        for p in range(5):
+            pass_ms[p] += evs[k][p].elapsed_time(evs[k][p + 1])
+        step_ms.append(evs[k][0].elapsed_time(evs[k][5]))
+    pass_ms = [t / K for t in pass_ms]
+    nf = len(fields)
+    ms_per_step = wall * 1e3 / K
+    vox = float(nz * ny * nx) * nf
+    value = vox * K / wall
+    pass_gbs = [plan.pass_bytes(p) * nf / (pass_ms[p] * 1e-3) / 1e9 for p in range(5)]
+    dom = int(np.argmax(pass_ms))
+    step_bytes = plan.info.bytes_per_field_tick * nf
+    mean_step_ms = float(np.mean(step_ms))
+
+    out = {
+        "metric": "voxel-updates/s for 8-field step at 1024x512x128 (% HBM roofline); p50/p99 latency",
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated lognormal background + 50 point sources; "
+                "random D, lambda per field)",
+        "config": {"workload": f"{nf} FP32 cytokine fields, {nx}x{ny}x{nz} patch grid, zero-padded "
+                               f"linear convolution on {2*nx}x{2*ny}x{2*nz} FFT box, Gaussian "
+                               "Green's kernel x decay, 1 tick per step",
+                   "fields": nf, "grid": [nx, ny, nz], "fft_box": [2 * nx, 2 * ny, 2 * nz]},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": vfd.PASS_NAMES[dom],
+            "achieved": pass_gbs[dom],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": pass_gbs[dom] / HBM_PEAK_GBS,
+            "traffic": None,
+            "step_frac": step_bytes / (mean_step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "passes_ms": dict(zip(vfd.PASS_NAMES, pass_ms)),
+            "passes_gbs": dict(zip(vfd.PASS_NAMES, pass_gbs)),
+            "algorithmic_bytes_per_step": step_bytes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(shape, args.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

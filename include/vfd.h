@@ -1,0 +1,125 @@
+/*
+ * vfd.h -- native C-ABI of the MI355X cytokine-diffusion engine.
+ *
+ * One diffusion tick of a 3D field u (row-major [nz][ny][nx], x fastest) is the
+ * operator of the reference's fftDiffuse3D (src/Diffusion/Diffusion.cu:1622-1934):
+ *
+ *     v = unpad( C2R( R2C(pad(u)) * K^ ) / (px*py*pz) )
+ *
+ * computed as five HBM passes (pad+R2C-x, C2C-y, C2C-z*K^*C2C-z^-1, C2C-y^-1,
+ * C2R-x+unpad).  Everything here is additive to the reference's interface:
+ * the reference-compatible entry points (c_ctx, fftDiffuse3D, ...) are in
+ * vf_diffusion.h and are implemented on top of these.
+ *
+ * Conventions: all device pointers are HIP device memory on the current
+ * device; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Functions return VFD_OK (0) or a negative VFD_ERR_*; vfd_last_error()
+ * returns the message of the calling thread's last failure.  Nothing in the
+ * library calls exit() or abort().
+ */
+#ifndef VFD_H
+#define VFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VFD_OK = 0,
+    VFD_ERR_INVALID = -1,     /* bad argument / dimensions */
+    VFD_ERR_UNSUPPORTED = -2, /* transform length not compiled in */
+    VFD_ERR_DEVICE = -3,      /* HIP runtime error */
+    VFD_ERR_NOMEM = -4,       /* device allocation failed */
+    VFD_ERR_BUDGET = -5,      /* HBM budget of an engine exceeded */
+    VFD_ERR_NOTFOUND = -6,    /* unknown run / ticket */
+    VFD_ERR_STATE = -7        /* call not valid in the current state */
+};
+
+enum { VFD_F32 = 4, VFD_F64 = 8 };
+/* pad modes: VFD_PAD_CONSTANT_VF is the reference's pConstantVF
+ * (src/enums.h:39, convolutionFFT.cuh:590-638); VFD_PAD_ZERO is the zero pad of
+ * a linear convolution. */
+enum { VFD_PAD_CONSTANT_VF = 3, VFD_PAD_ZERO = 100 };
+/* kernel-spectrum formats:
+ *  VFD_KSPEC_REAL        real K^ of an even kernel, [pz][py][ld] (vfd_plan_info ld)
+ *  VFD_KSPEC_COMPLEX_STD complex K^ in the R2C layout [pz][py][px/2+1] of the
+ *                        reference (d_kernelspectrum_h, Diffusion.cu:1605) */
+enum { VFD_KSPEC_REAL = 0, VFD_KSPEC_COMPLEX_STD = 1 };
+
+typedef struct vfd_plan vfd_plan;
+
+typedef struct {
+    int nz, ny, nx;  /* data box */
+    int pz, py, px;  /* FFT box; 0 means 2*n (zero-padded linear convolution) */
+    int precision;   /* VFD_F32 or VFD_F64 */
+    int pad_mode;    /* VFD_PAD_ZERO or VFD_PAD_CONSTANT_VF */
+    int kspec_format;/* VFD_KSPEC_REAL or VFD_KSPEC_COMPLEX_STD */
+} vfd_plan_desc;
+
+typedef struct {
+    int nz, ny, nx, pz, py, px;
+    int L;              /* px/2: packed row length of the x passes */
+    int hx;             /* px/2+1: half-spectrum row length */
+    long ld;            /* work row stride in complex elements */
+    int rs_y, rs_z;     /* even/odd split used on y and z (1 or 2) */
+    size_t work_bytes;  /* per-field work buffer */
+    size_t kspec_bytes; /* per-field kernel spectrum in the plan's format */
+    double bytes_per_field_tick; /* algorithmic HBM bytes of one tick (pruned model) */
+} vfd_plan_info;
+
+typedef struct {
+    const void* in;    /* real field [nz][ny][nx] */
+    void* out;         /* real field [nz][ny][nx]; may equal in */
+    void* work;        /* >= work_bytes, private to this field during the call */
+    const void* kspec; /* kernel spectrum in the plan's format */
+    double base;       /* constant-pad value (VFD_PAD_CONSTANT_VF) */
+    int epi;           /* epithelium index: interior x < epi (VFD_PAD_CONSTANT_VF) */
+    int reserved;
+} vfd_field;
+
+const char* vfd_last_error(void);
+int vfd_version(void);
+/* 1 when a transform of length q is compiled in. */
+int vfd_length_supported(int q);
+
+int vfd_plan_create(const vfd_plan_desc* desc, vfd_plan** plan);
+void vfd_plan_destroy(vfd_plan* plan);
+int vfd_plan_get_info(const vfd_plan* plan, vfd_plan_info* info);
+
+/* One tick for nfields fields of this plan, asynchronous on stream.  Fields are
+ * batched into launches of up to 32. */
+int vfd_diffuse(const vfd_plan* plan, const vfd_field* fields, int nfields, void* stream);
+
+/* The five passes of vfd_diffuse, one at a time (same arguments), in order
+ * VFD_PASS_XFWD, VFD_PASS_YFWD, VFD_PASS_ZCONV, VFD_PASS_YINV, VFD_PASS_XINV;
+ * for per-pass timing.  vfd_plan_pass_bytes gives the algorithmic HBM bytes
+ * of one pass for one field (sum over passes = bytes_per_field_tick). */
+enum { VFD_PASS_XFWD = 0, VFD_PASS_YFWD = 1, VFD_PASS_ZCONV = 2, VFD_PASS_YINV = 3, VFD_PASS_XINV = 4 };
+int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream);
+double vfd_plan_pass_bytes(const vfd_plan* plan, int pass);
+
+/* Real kernel spectrum (VFD_KSPEC_REAL) of the normalized sampled Gaussian
+ * g(j) ~ exp(-j^2/(2 sigma2)), |j| <= ceil(6 sqrt(sigma2)) per axis, times decay,
+ * computed in FP64 and stored in the plan's precision. */
+int vfd_gauss_spectrum(const vfd_plan* plan, double sigma2, double decay, void* d_kspec, void* stream);
+
+/* Compensated (Neumaier, FP64) sum of each of nfields real fields of the plan's
+ * data box; fixed reduction order (bitwise reproducible).  Synchronizes stream. */
+int vfd_field_mass(const vfd_plan* plan, const void* const* d_fields, int nfields, double* mass,
+                   void* stream);
+
+/* Standalone 3D transforms on a full box (px,py,pz of the plan, data box = box):
+ * R2C real [pz][py][px] -> complex R2C layout [pz][py][px/2+1]; C2R the inverse
+ * (unnormalized).  d_out / d_in of the complex side must hold
+ * pz*py*(px/2+1) complex elements.  Plans for these must have n == p. */
+int vfd_r2c_3d(const vfd_plan* plan, const void* d_real, void* d_complex, void* stream);
+int vfd_c2r_3d(const vfd_plan* plan, void* d_complex, void* d_real, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFD_H */

@@ -1,0 +1,38 @@
+// vfd_internal.h -- host-side plan and helpers shared by the runtime files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "../../include/vfd.h"
+#include "vfd_types.h"
+
+namespace vfd {
+
+struct Plan {
+    int nz, ny, nx, pz, py, px;
+    int prec, pad_mode, kspec_format;
+    int L, hx;
+    int rs_y, rs_z, qy, qz;
+    bool full_box;
+    long ld, plane;
+    long ks_row, ks_plane;
+    size_t work_bytes, kspec_bytes;
+    double bytes_per_field_tick;
+    const void *tw_x, *tw_y, *tw_z;
+};
+
+int set_error(int code, const char* fmt, ...);
+int hip_check(hipError_t e, const char* what);
+int twiddles(int p, int prec, const void** out);
+
+int plan_init(const vfd_plan_desc* d, Plan* p);
+int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
+int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st);
+double plan_pass_bytes(const Plan& p, int pass);
+int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspec, hipStream_t st);
+int plan_mass(const Plan& p, const void* const* d_fields, int nfields, double* mass, hipStream_t st);
+int plan_r2c(const Plan& p, const void* in, void* out, hipStream_t st);
+int plan_c2r(const Plan& p, void* in, void* out, hipStream_t st);
+
+}  // namespace vfd

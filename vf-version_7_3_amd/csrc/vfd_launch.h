@@ -1,0 +1,24 @@
+// vfd_launch.h -- host entry points of the templated passes (one per real type).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "vfd_types.h"
+
+namespace vfd {
+
+// True when a transform of this length is compiled into the library.
+bool length_supported(int q);
+
+// x-forward (pad + R2C) over Ny*Nz rows of nfields fields; L = Px/2.
+template <typename T>
+hipError_t launch_xfwd(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
+                       const void* tw, hipStream_t st);
+// x-inverse (C2R + unpad).
+template <typename T>
+hipError_t launch_xinv(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
+                       const void* tw, hipStream_t st);
+// column pass of sub-length Q (P = Q * cd.rs) over nplanes planes.
+template <typename T>
+hipError_t launch_col(int Q, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
+                      const void* tw, hipStream_t st);
+
+}  // namespace vfd

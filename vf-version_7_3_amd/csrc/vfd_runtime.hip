@@ -1,0 +1,526 @@
+// vfd_runtime.hip -- plans, twiddle tables, kernel spectra, mass sums and the
+// native C-ABI (include/vfd.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vfd.h"
+#include "vfd_internal.h"
+#include "vfd_launch.h"
+
+namespace vfd {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return VFD_OK;
+    return set_error(e == hipErrorOutOfMemory ? VFD_ERR_NOMEM : VFD_ERR_DEVICE, "%s: %s", what,
+                     hipGetErrorString(e));
+}
+
+// --------------------------------------------------------- twiddle tables
+// W_P^k = exp(-2 pi i k / P), k in [0, P), computed in long double on the host,
+// one table per (device, P, precision), kept for the life of the process.
+struct TwKey {
+    int dev, p, prec;
+    bool operator<(const TwKey& o) const {
+        return dev != o.dev ? dev < o.dev : (p != o.p ? p < o.p : prec < o.prec);
+    }
+};
+static std::mutex g_tw_mu;
+static std::map<TwKey, void*> g_tw;
+
+int twiddles(int p, int prec, const void** out) {
+    int dev = 0;
+    if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_tw.find({dev, p, prec});
+    if (it != g_tw.end()) {
+        *out = it->second;
+        return VFD_OK;
+    }
+    const size_t bytes = (size_t)p * 2 * prec;
+    std::vector<unsigned char> host(bytes);
+    const long double tp = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < p; ++k) {
+        long double a = tp * (long double)k / (long double)p;
+        double c = (double)cosl(a), s = (double)-sinl(a);
+        if (prec == VFD_F32) {
+            float* f = (float*)host.data();
+            f[2 * k] = (float)c;
+            f[2 * k + 1] = (float)s;
+        } else {
+            double* d = (double*)host.data();
+            d[2 * k] = c;
+            d[2 * k + 1] = s;
+        }
+    }
+    void* d = nullptr;
+    if (int rc = hip_check(hipMalloc(&d, bytes), "hipMalloc(twiddles)")) return rc;
+    if (int rc = hip_check(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice), "twiddle upload")) {
+        (void)hipFree(d);
+        return rc;
+    }
+    g_tw[{dev, p, prec}] = d;
+    *out = d;
+    return VFD_OK;
+}
+
+// ------------------------------------------------------------------ plans
+static long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+int plan_init(const vfd_plan_desc* d, Plan* p) {
+    if (!d) return set_error(VFD_ERR_INVALID, "plan: null descriptor");
+    Plan q{};
+    q.nz = d->nz; q.ny = d->ny; q.nx = d->nx;
+    q.pz = d->pz ? d->pz : 2 * d->nz;
+    q.py = d->py ? d->py : 2 * d->ny;
+    q.px = d->px ? d->px : 2 * d->nx;
+    q.prec = d->precision;
+    q.pad_mode = d->pad_mode;
+    q.kspec_format = d->kspec_format;
+    if (q.nx < 1 || q.ny < 1 || q.nz < 1)
+        return set_error(VFD_ERR_INVALID, "plan: data box %dx%dx%d must be positive", q.nx, q.ny, q.nz);
+    if (q.px < q.nx || q.py < q.ny || q.pz < q.nz)
+        return set_error(VFD_ERR_INVALID, "plan: FFT box %dx%dx%d smaller than data box %dx%dx%d", q.px, q.py,
+                         q.pz, q.nx, q.ny, q.nz);
+    if (q.px % 2)
+        return set_error(VFD_ERR_UNSUPPORTED, "plan: px=%d must be even (R2C via half-length complex FFT)", q.px);
+    if (q.prec != VFD_F32 && q.prec != VFD_F64)
+        return set_error(VFD_ERR_INVALID, "plan: precision must be VFD_F32 or VFD_F64");
+    if (q.pad_mode != VFD_PAD_ZERO && q.pad_mode != VFD_PAD_CONSTANT_VF)
+        return set_error(VFD_ERR_INVALID, "plan: unknown pad mode %d", q.pad_mode);
+    if (q.kspec_format != VFD_KSPEC_REAL && q.kspec_format != VFD_KSPEC_COMPLEX_STD)
+        return set_error(VFD_ERR_INVALID, "plan: unknown kernel-spectrum format %d", q.kspec_format);
+    q.L = q.px / 2;
+    q.hx = q.L + 1;
+    if (!length_supported(q.L))
+        return set_error(VFD_ERR_UNSUPPORTED, "plan: x transform length %d (px/2) not compiled in", q.L);
+    auto pick = [](int n, int p, int* rs, int* qq) -> bool {
+        if (p % 2 == 0 && n <= p / 2 && length_supported(p / 2)) {
+            *rs = 2;
+            *qq = p / 2;
+            return true;
+        }
+        *rs = 1;
+        *qq = p;
+        return length_supported(p);
+    };
+    if (!pick(q.ny, q.py, &q.rs_y, &q.qy))
+        return set_error(VFD_ERR_UNSUPPORTED, "plan: y transform length %d not compiled in", q.py);
+    if (!pick(q.nz, q.pz, &q.rs_z, &q.qz))
+        return set_error(VFD_ERR_UNSUPPORTED, "plan: z transform length %d not compiled in", q.pz);
+    q.full_box = (q.nx == q.px && q.ny == q.py && q.nz == q.pz);
+    q.ld = q.full_box ? q.hx : round_up(q.hx, 16);
+    q.plane = (long)q.py * q.ld;
+    const size_t cxb = 2 * (size_t)q.prec;
+    q.work_bytes = (size_t)q.nz * q.plane * cxb;
+    if (q.kspec_format == VFD_KSPEC_REAL) {
+        q.ks_row = (long)q.py * q.ld;  // along kz
+        q.ks_plane = q.ld;             // along ky
+        q.kspec_bytes = (size_t)q.pz * q.py * q.ld * q.prec;
+    } else {
+        q.ks_row = (long)q.py * q.hx;
+        q.ks_plane = q.hx;
+        q.kspec_bytes = (size_t)q.pz * q.py * q.hx * cxb;
+    }
+    // pruned 5-pass algorithmic bytes: 8N + 32 A1 + 32 A2 + 4 A3 (FP32); the
+    // complex terms scale with the precision and K^ with its element size.
+    const double N = (double)q.nx * q.ny * q.nz;
+    const double A1 = (double)q.hx * q.ny * q.nz, A2 = (double)q.hx * q.py * q.nz,
+                 A3 = (double)q.hx * q.py * q.pz;
+    const double r = (double)q.prec, c = 2.0 * q.prec;
+    const double kel = q.kspec_format == VFD_KSPEC_REAL ? r : c;
+    q.bytes_per_field_tick = 2 * r * N + 4 * c * A1 + 4 * c * A2 + kel * A3;
+    if (int rc = twiddles(q.px, q.prec, &q.tw_x)) return rc;
+    if (int rc = twiddles(q.py, q.prec, &q.tw_y)) return rc;
+    if (int rc = twiddles(q.pz, q.prec, &q.tw_z)) return rc;
+    *p = q;
+    return VFD_OK;
+}
+
+// ------------------------------------------------------------- execution
+template <typename T>
+static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pass_lo, int pass_hi,
+                    hipStream_t st) {
+    for (int f0 = 0; f0 < nfields; f0 += kMaxFieldsPerLaunch) {
+        const int nf = std::min(kMaxFieldsPerLaunch, nfields - f0);
+        FieldBatch fb;
+        memset(&fb, 0, sizeof fb);
+        for (int i = 0; i < nf; ++i) {
+            const vfd_field& f = fields[f0 + i];
+            fb.f[i].in = f.in;
+            fb.f[i].out = f.out;
+            fb.f[i].work = f.work;
+            fb.f[i].kspec = f.kspec;
+            fb.f[i].base = f.base;
+            fb.f[i].epi = f.epi;
+        }
+        GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD};
+        ColDesc y{};
+        y.mode = COL_FWD; y.rs = p.rs_y; y.nin = p.ny; y.nout = p.py; y.ncols = p.hx;
+        y.rstride = p.ld; y.pstride = p.plane; y.scale = 1.0;
+        ColDesc z{};
+        z.mode = COL_FUSED; z.rs = p.rs_z; z.nin = p.nz; z.nout = p.nz; z.ncols = p.hx;
+        z.rstride = p.plane; z.pstride = p.ld; z.ks_row = p.ks_row; z.ks_plane = p.ks_plane;
+        z.kspec_complex = p.kspec_format == VFD_KSPEC_COMPLEX_STD;
+        z.scale = 1.0 / ((double)p.px * p.py * p.pz);
+        ColDesc yi = y;
+        yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
+        for (int pass = pass_lo; pass < pass_hi; ++pass) {
+            hipError_t e = hipSuccess;
+            switch (pass) {
+                case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st); break;
+                case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st); break;
+                case VFD_PASS_ZCONV: e = launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st); break;
+                case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st); break;
+                case VFD_PASS_XINV: e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st); break;
+            }
+            if (e != hipSuccess) return hip_check(e, "pass launch");
+        }
+    }
+    return VFD_OK;
+}
+
+static int check_fields(const vfd_field* fields, int nfields) {
+    if (nfields < 0 || (nfields > 0 && !fields)) return set_error(VFD_ERR_INVALID, "diffuse: bad field list");
+    for (int i = 0; i < nfields; ++i)
+        if (!fields[i].in || !fields[i].out || !fields[i].work || !fields[i].kspec)
+            return set_error(VFD_ERR_INVALID, "diffuse: field %d has a null pointer", i);
+    return VFD_OK;
+}
+
+int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st) {
+    if (int rc = check_fields(fields, nfields)) return rc;
+    if (lo < 0 || hi > 5 || lo >= hi) return set_error(VFD_ERR_INVALID, "diffuse: bad pass range [%d,%d)", lo, hi);
+    if (nfields == 0) return VFD_OK;
+    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, lo, hi, st)
+                             : run_tick<double>(p, fields, nfields, lo, hi, st);
+}
+
+double plan_pass_bytes(const Plan& p, int pass) {
+    const double N = (double)p.nx * p.ny * p.nz;
+    const double A1 = (double)p.hx * p.ny * p.nz, A2 = (double)p.hx * p.py * p.nz,
+                 A3 = (double)p.hx * p.py * p.pz;
+    const double r = (double)p.prec, c = 2.0 * p.prec;
+    const double kel = p.kspec_format == VFD_KSPEC_REAL ? r : c;
+    switch (pass) {
+        case VFD_PASS_XFWD: return r * N + c * A1;
+        case VFD_PASS_YFWD: return c * A1 + c * A2;
+        case VFD_PASS_ZCONV: return 2 * c * A2 + kel * A3;
+        case VFD_PASS_YINV: return c * A2 + c * A1;
+        case VFD_PASS_XINV: return c * A1 + r * N;
+        default: return 0.0;
+    }
+}
+
+int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st) {
+    return plan_diffuse_passes(p, fields, nfields, 0, 5, st);
+}
+
+// Standalone full-box transforms (used by the kernel-spectrum precompute).
+template <typename T>
+static int run_r2c(const Plan& p, const void* in, void* out, hipStream_t st) {
+    FieldBatch fb;
+    memset(&fb, 0, sizeof fb);
+    fb.f[0].in = in;
+    fb.f[0].work = out;
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0};
+    hipError_t e = launch_xfwd<T>(p.L, fb, 1, g, p.tw_x, st);
+    if (e != hipSuccess) return hip_check(e, "r2c x launch");
+    ColDesc y{};
+    y.mode = COL_FWD; y.rs = 1; y.nin = p.ny; y.nout = p.py; y.ncols = p.hx;
+    y.rstride = p.ld; y.pstride = p.plane; y.scale = 1.0;
+    e = launch_col<T>(p.py, fb, 1, y, p.nz, p.tw_y, st);
+    if (e != hipSuccess) return hip_check(e, "r2c y launch");
+    ColDesc z = y;
+    z.nin = p.nz; z.nout = p.pz; z.rstride = p.plane; z.pstride = p.ld;
+    e = launch_col<T>(p.pz, fb, 1, z, p.py, p.tw_z, st);
+    return e == hipSuccess ? VFD_OK : hip_check(e, "r2c z launch");
+}
+
+template <typename T>
+static int run_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
+    FieldBatch fb;
+    memset(&fb, 0, sizeof fb);
+    fb.f[0].out = out;
+    fb.f[0].work = in;
+    ColDesc z{};
+    z.mode = COL_INV; z.rs = 1; z.nin = p.pz; z.nout = p.pz; z.ncols = p.hx;
+    z.rstride = p.plane; z.pstride = p.ld; z.scale = 1.0;
+    hipError_t e = launch_col<T>(p.pz, fb, 1, z, p.py, p.tw_z, st);
+    if (e != hipSuccess) return hip_check(e, "c2r z launch");
+    ColDesc y = z;
+    y.nin = p.py; y.nout = p.py; y.rstride = p.ld; y.pstride = p.plane;
+    e = launch_col<T>(p.py, fb, 1, y, p.nz, p.tw_y, st);
+    if (e != hipSuccess) return hip_check(e, "c2r y launch");
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0};
+    e = launch_xinv<T>(p.L, fb, 1, g, p.tw_x, st);
+    return e == hipSuccess ? VFD_OK : hip_check(e, "c2r x launch");
+}
+
+// --------------------------------------------------- Gaussian spectrum
+// K^(kz,ky,kx) = decay * Gz(kz) Gy(ky) Gx(kx), G(k) = sum_{|j|<=R} g_j cos(2 pi j k / P):
+// the DFT of the centred, normalized, sampled Gaussian wrapped to the origin.
+template <typename T>
+__global__ void gauss_product_kernel(T* __restrict__ out, const double* __restrict__ gx,
+                                     const double* __restrict__ gy, const double* __restrict__ gz,
+                                     int hx, long ld, int py, int pz, double decay) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)pz * py * ld;
+    if (i >= total) return;
+    const long kx = i % ld;
+    const long r = i / ld;
+    const int ky = (int)(r % py), kz = (int)(r / py);
+    out[i] = kx < hx ? (T)(decay * gz[kz] * gy[ky] * gx[kx]) : T(0);
+}
+
+static void gauss_dft(double sigma2, int p, int nk, std::vector<double>& out) {
+    const int R = (int)std::ceil(6.0 * std::sqrt(sigma2));
+    std::vector<double> g(2 * R + 1);
+    double s = 0.0;
+    for (int j = -R; j <= R; ++j) s += (g[j + R] = std::exp(-(double)j * j / (2.0 * sigma2)));
+    for (auto& v : g) v /= s;
+    out.assign(nk, 0.0);
+    const long double tp = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < nk; ++k) {
+        long double acc = 0.0L;
+        for (int j = -R; j <= R; ++j) {
+            long e = (((long)j * k) % p + p) % p;
+            acc += (long double)g[j + R] * cosl(tp * (long double)e / (long double)p);
+        }
+        out[k] = (double)acc;
+    }
+}
+
+int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspec, hipStream_t st) {
+    if (p.kspec_format != VFD_KSPEC_REAL)
+        return set_error(VFD_ERR_INVALID, "gauss spectrum: plan kernel-spectrum format must be VFD_KSPEC_REAL");
+    if (!(sigma2 > 0.0) || !d_kspec) return set_error(VFD_ERR_INVALID, "gauss spectrum: sigma2 must be > 0");
+    const int R = (int)std::ceil(6.0 * std::sqrt(sigma2));
+    if (R >= p.px || R >= p.py || R >= p.pz)
+        return set_error(VFD_ERR_INVALID, "gauss spectrum: kernel radius %d does not fit the FFT box", R);
+    std::vector<double> gx, gy, gz;
+    gauss_dft(sigma2, p.px, p.hx, gx);
+    gauss_dft(sigma2, p.py, p.py, gy);
+    gauss_dft(sigma2, p.pz, p.pz, gz);
+    double* d = nullptr;
+    const size_t n = gx.size() + gy.size() + gz.size();
+    if (int rc = hip_check(hipMallocAsync((void**)&d, n * sizeof(double), st), "hipMallocAsync(gauss)")) return rc;
+    std::vector<double> all;
+    all.insert(all.end(), gx.begin(), gx.end());
+    all.insert(all.end(), gy.begin(), gy.end());
+    all.insert(all.end(), gz.begin(), gz.end());
+    int rc = hip_check(hipMemcpyAsync(d, all.data(), n * sizeof(double), hipMemcpyHostToDevice, st), "gauss upload");
+    if (!rc) {
+        const long total = (long)p.pz * p.py * p.ld;
+        const unsigned blocks = (unsigned)((total + 255) / 256);
+        if (p.prec == VFD_F32)
+            hipLaunchKernelGGL(gauss_product_kernel<float>, dim3(blocks), dim3(256), 0, st, (float*)d_kspec, d,
+                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, decay);
+        else
+            hipLaunchKernelGGL(gauss_product_kernel<double>, dim3(blocks), dim3(256), 0, st, (double*)d_kspec, d,
+                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, decay);
+        rc = hip_check(hipGetLastError(), "gauss spectrum launch");
+        // the host vector must outlive the async copy
+        if (!rc) rc = hip_check(hipStreamSynchronize(st), "gauss spectrum sync");
+    }
+    (void)hipFreeAsync(d, st);
+    return rc;
+}
+
+// ------------------------------------------------------ compensated mass
+template <typename T>
+__device__ __forceinline__ void neumaier_add(double& s, double& c, T xv) {
+    const double x = (double)xv;
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+// Each block sums a fixed contiguous chunk of one field; lanes accumulate
+// strided elements, then a fixed-order tree combines (s, c) pairs.
+template <typename T>
+__global__ void mass_partial_kernel(const T* const* __restrict__ fields, long n, long chunk,
+                                    double2* __restrict__ partial) {
+    const T* f = fields[blockIdx.y];
+    const long lo = (long)blockIdx.x * chunk;
+    const long hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0, c = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) neumaier_add(s, c, f[i]);
+    __shared__ double ss[256], sc[256];
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double a = ss[threadIdx.x], ca = sc[threadIdx.x];
+            double b = ss[threadIdx.x + w], cb = sc[threadIdx.x + w];
+            double t = a + b;
+            double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+            ss[threadIdx.x] = t;
+            sc[threadIdx.x] = ca + cb + e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = double2{ss[0], sc[0]};
+}
+
+__global__ void mass_final_kernel(const double2* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    // one thread per field, fixed order
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (int)gridDim.x * (int)blockDim.x) return;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblocks; ++b) {
+        const double2 p = partial[(long)f * nblocks + b];
+        neumaier_add(s, c, p.x);
+        c += p.y;
+    }
+    out[f] = s + c;
+}
+
+int plan_mass(const Plan& p, const void* const* d_fields, int nfields, double* mass, hipStream_t st) {
+    if (nfields <= 0 || !d_fields || !mass) return set_error(VFD_ERR_INVALID, "mass: bad arguments");
+    const long n = (long)p.nx * p.ny * p.nz;
+    const int nblocks = (int)std::min<long>(1024, (n + 4095) / 4096);
+    const long chunk = (n + nblocks - 1) / nblocks;
+    const size_t ptr_bytes = sizeof(void*) * nfields;
+    const size_t part_bytes = sizeof(double2) * (size_t)nblocks * nfields;
+    const size_t out_bytes = sizeof(double) * nfields;
+    char* d = nullptr;
+    if (int rc = hip_check(hipMallocAsync((void**)&d, ptr_bytes + part_bytes + out_bytes + 64, st), "hipMallocAsync(mass)"))
+        return rc;
+    void** dptr = (void**)d;
+    double2* dpart = (double2*)(d + ((ptr_bytes + 15) / 16) * 16);
+    double* dout = (double*)((char*)dpart + part_bytes);
+    int rc = hip_check(hipMemcpyAsync(dptr, d_fields, ptr_bytes, hipMemcpyHostToDevice, st), "mass upload");
+    if (!rc) {
+        if (p.prec == VFD_F32)
+            hipLaunchKernelGGL(mass_partial_kernel<float>, dim3(nblocks, nfields), dim3(256), 0, st,
+                               (const float* const*)dptr, n, chunk, dpart);
+        else
+            hipLaunchKernelGGL(mass_partial_kernel<double>, dim3(nblocks, nfields), dim3(256), 0, st,
+                               (const double* const*)dptr, n, chunk, dpart);
+        hipLaunchKernelGGL(mass_final_kernel, dim3(nfields), dim3(1), 0, st, dpart, nblocks, dout);
+        rc = hip_check(hipGetLastError(), "mass launch");
+    }
+    if (!rc) rc = hip_check(hipMemcpyAsync(mass, dout, out_bytes, hipMemcpyDeviceToHost, st), "mass download");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "mass sync");
+    (void)hipFreeAsync(d, st);
+    return rc;
+}
+
+int plan_r2c(const Plan& p, const void* in, void* out, hipStream_t st) {
+    if (!p.full_box) return set_error(VFD_ERR_INVALID, "r2c: plan data box must equal its FFT box");
+    return p.prec == VFD_F32 ? run_r2c<float>(p, in, out, st) : run_r2c<double>(p, in, out, st);
+}
+int plan_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
+    if (!p.full_box) return set_error(VFD_ERR_INVALID, "c2r: plan data box must equal its FFT box");
+    return p.prec == VFD_F32 ? run_c2r<float>(p, in, out, st) : run_c2r<double>(p, in, out, st);
+}
+
+bool length_supported(int q) {
+    switch (q) {
+#define VFD_CASE(Q) case Q:
+        VFD_CASE(2) VFD_CASE(4) VFD_CASE(8) VFD_CASE(16) VFD_CASE(32) VFD_CASE(64) VFD_CASE(128)
+        VFD_CASE(256) VFD_CASE(512) VFD_CASE(1024) VFD_CASE(2048) VFD_CASE(3) VFD_CASE(6) VFD_CASE(12)
+        VFD_CASE(24) VFD_CASE(48) VFD_CASE(96) VFD_CASE(192) VFD_CASE(384) VFD_CASE(768) VFD_CASE(1536)
+        VFD_CASE(9) VFD_CASE(18) VFD_CASE(36) VFD_CASE(72) VFD_CASE(144) VFD_CASE(288) VFD_CASE(576)
+        VFD_CASE(1152) VFD_CASE(2304) VFD_CASE(616)
+#undef VFD_CASE
+        return true;
+        default:
+            return false;
+    }
+}
+
+}  // namespace vfd
+
+// ====================================================================== C-ABI
+using namespace vfd;
+
+struct vfd_plan {
+    Plan p;
+};
+
+extern "C" {
+
+const char* vfd_last_error(void) { return g_last_error.c_str(); }
+int vfd_version(void) { return 10000; }
+int vfd_length_supported(int q) { return length_supported(q) ? 1 : 0; }
+
+int vfd_plan_create(const vfd_plan_desc* desc, vfd_plan** plan) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "plan_create: null output");
+    *plan = nullptr;
+    Plan p;
+    if (int rc = plan_init(desc, &p)) return rc;
+    *plan = new vfd_plan{p};
+    return VFD_OK;
+}
+
+void vfd_plan_destroy(vfd_plan* plan) { delete plan; }
+
+int vfd_plan_get_info(const vfd_plan* plan, vfd_plan_info* info) {
+    if (!plan || !info) return set_error(VFD_ERR_INVALID, "plan_get_info: null argument");
+    const Plan& p = plan->p;
+    info->nz = p.nz; info->ny = p.ny; info->nx = p.nx;
+    info->pz = p.pz; info->py = p.py; info->px = p.px;
+    info->L = p.L; info->hx = p.hx; info->ld = p.ld;
+    info->rs_y = p.rs_y; info->rs_z = p.rs_z;
+    info->work_bytes = p.work_bytes;
+    info->kspec_bytes = p.kspec_bytes;
+    info->bytes_per_field_tick = p.bytes_per_field_tick;
+    return VFD_OK;
+}
+
+int vfd_diffuse(const vfd_plan* plan, const vfd_field* fields, int nfields, void* stream) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "diffuse: null plan");
+    return plan_diffuse(plan->p, fields, nfields, (hipStream_t)stream);
+}
+
+int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "diffuse_pass: null plan");
+    return plan_diffuse_passes(plan->p, fields, nfields, pass, pass + 1, (hipStream_t)stream);
+}
+
+double vfd_plan_pass_bytes(const vfd_plan* plan, int pass) { return plan ? plan_pass_bytes(plan->p, pass) : 0.0; }
+
+int vfd_gauss_spectrum(const vfd_plan* plan, double sigma2, double decay, void* d_kspec, void* stream) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "gauss_spectrum: null plan");
+    return plan_gauss_spectrum(plan->p, sigma2, decay, d_kspec, (hipStream_t)stream);
+}
+
+int vfd_field_mass(const vfd_plan* plan, const void* const* d_fields, int nfields, double* mass, void* stream) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "field_mass: null plan");
+    return plan_mass(plan->p, d_fields, nfields, mass, (hipStream_t)stream);
+}
+
+int vfd_r2c_3d(const vfd_plan* plan, const void* d_real, void* d_complex, void* stream) {
+    if (!plan || !d_real || !d_complex) return set_error(VFD_ERR_INVALID, "r2c_3d: null argument");
+    return plan_r2c(plan->p, d_real, d_complex, (hipStream_t)stream);
+}
+
+int vfd_c2r_3d(const vfd_plan* plan, void* d_complex, void* d_real, void* stream) {
+    if (!plan || !d_real || !d_complex) return set_error(VFD_ERR_INVALID, "c2r_3d: null argument");
+    return plan_c2r(plan->p, d_complex, d_real, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// vfd_types.h -- descriptors shared by the host runtime and the device passes.
+#pragma once
+
+namespace vfd {
+
+// pad modes: the reference's pad_t (src/enums.h:35-40) value for the constant
+// pad, plus the zero pad of the linear-convolution configuration.
+enum PadMode { PAD_CONSTANT_VF = 3, PAD_ZERO = 100 };
+
+enum ColMode { COL_FWD = 0, COL_INV = 1, COL_FUSED = 2 };
+
+// Per-field pointers of one launch.
+struct FieldDesc {
+    const void* in;     // real field [Nz][Ny][Nx] (x-forward input)
+    void* out;          // real field [Nz][Ny][Nx] (x-inverse output; may alias in)
+    void* work;         // complex work [planes][Py][ld]
+    const void* kspec;  // kernel spectrum (real or complex), see ColDesc strides
+    double base;        // constant pad value (PAD_CONSTANT_VF), else 0
+    int epi;            // epithelium index: interior x < epi (PAD_CONSTANT_VF)
+    int pad_;
+};
+
+// Fields of one launch, passed by value as a kernel argument (graph-capture
+// friendly, no device-side descriptor copies).  blockIdx.z selects the field.
+constexpr int kMaxFieldsPerLaunch = 32;
+struct FieldBatch {
+    FieldDesc f[kMaxFieldsPerLaunch];
+};
+
+// Shape of the x passes (identical for every field of a launch).
+struct GridDesc {
+    int Nx, Ny, Nz;     // data box
+    int L;              // Px / 2 (packed complex row length)
+    long ld;            // work row stride (complex elements), >= L + 1
+    long plane;         // work plane stride = Py * ld
+    int pad_mode;       // PadMode
+    int kspec_complex;  // 1: complex spectrum, 0: real
+};
+
+// One column pass (y or z).
+struct ColDesc {
+    int mode;           // ColMode
+    int rs;             // 1 or 2 (even/odd split of the padded length)
+    int nin, nout;      // valid input rows / rows written back
+    long ncols;         // columns per plane (= L + 1)
+    long rstride;       // element stride between rows of a column
+    long pstride;       // element stride between planes (blockIdx.y)
+    long ks_row;        // kernel-spectrum stride along this axis' frequency
+    long ks_plane;      // kernel-spectrum stride along the plane index
+    int kspec_complex;
+    double scale;       // applied to K^ products (1 / (Px Py Pz))
+};
+
+}  // namespace vfd

@@ -1,0 +1,178 @@
+"""Python host binding of libvfd.so (the MI355X cytokine-diffusion engine).
+
+Thin ctypes layer over the C-ABI in include/vfd.h and include/vf_diffusion.h.
+Device memory and streams come from torch (plumbing only): every compute call
+goes to the HIP kernels in libvfd.so, and importing this module fails loudly
+when the library has not been built.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libvfd.so")
+
+VFD_OK = 0
+VFD_ERR_INVALID, VFD_ERR_UNSUPPORTED, VFD_ERR_DEVICE = -1, -2, -3
+VFD_ERR_NOMEM, VFD_ERR_BUDGET, VFD_ERR_NOTFOUND, VFD_ERR_STATE = -4, -5, -6, -7
+F32, F64 = 4, 8
+PAD_CONSTANT_VF, PAD_ZERO = 3, 100
+KSPEC_REAL, KSPEC_COMPLEX_STD = 0, 1
+PASS_XFWD, PASS_YFWD, PASS_ZCONV, PASS_YINV, PASS_XINV = 0, 1, 2, 3, 4
+PASS_NAMES = ["xfwd", "yfwd", "zconv", "yinv", "xinv"]
+
+
+class VfdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vfd error {code}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libvfd.so not built at {LIB_PATH}: run __graft_entry__.build() "
+                      "(make -C vf-version_7_3_amd)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_i, _d, _p, _sz = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("nz", _i), ("ny", _i), ("nx", _i), ("pz", _i), ("py", _i), ("px", _i),
+                ("precision", _i), ("pad_mode", _i), ("kspec_format", _i)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("nz", _i), ("ny", _i), ("nx", _i), ("pz", _i), ("py", _i), ("px", _i),
+                ("L", _i), ("hx", _i), ("ld", ctypes.c_long), ("rs_y", _i), ("rs_z", _i),
+                ("work_bytes", _sz), ("kspec_bytes", _sz), ("bytes_per_field_tick", _d)]
+
+
+class Field(ctypes.Structure):
+    _fields_ = [("inp", _p), ("out", _p), ("work", _p), ("kspec", _p), ("base", _d),
+                ("epi", _i), ("reserved", _i)]
+
+
+lib.vfd_last_error.restype = ctypes.c_char_p
+lib.vfd_version.restype = _i
+lib.vfd_length_supported.argtypes = [_i]
+lib.vfd_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(_p)]
+lib.vfd_plan_destroy.argtypes = [_p]
+lib.vfd_plan_destroy.restype = None
+lib.vfd_plan_get_info.argtypes = [_p, ctypes.POINTER(PlanInfo)]
+lib.vfd_diffuse.argtypes = [_p, ctypes.POINTER(Field), _i, _p]
+lib.vfd_diffuse_pass.argtypes = [_p, ctypes.POINTER(Field), _i, _i, _p]
+lib.vfd_plan_pass_bytes.argtypes = [_p, _i]
+lib.vfd_plan_pass_bytes.restype = _d
+lib.vfd_gauss_spectrum.argtypes = [_p, _d, _d, _p, _p]
+lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _p]
+lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
+lib.vfd_c2r_3d.argtypes = [_p, _p, _p, _p]
+
+
+def check(rc):
+    if rc != VFD_OK:
+        raise VfdError(rc, lib.vfd_last_error().decode())
+    return rc
+
+
+def last_error():
+    return lib.vfd_last_error().decode()
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class Plan:
+    """One grid shape: data box (nz, ny, nx), FFT box (pz, py, px)."""
+
+    def __init__(self, shape, box=None, precision=F32, pad_mode=PAD_ZERO,
+                 kspec_format=KSPEC_REAL):
+        nz, ny, nx = shape
+        pz, py, px = box if box is not None else (0, 0, 0)
+        self.desc = PlanDesc(nz, ny, nx, pz, py, px, precision, pad_mode, kspec_format)
+        h = ctypes.c_void_p()
+        check(lib.vfd_plan_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.h = h
+        self.info = PlanInfo()
+        check(lib.vfd_plan_get_info(self.h, ctypes.byref(self.info)))
+        self.precision = precision
+        self.kspec_format = kspec_format
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.vfd_plan_destroy(self.h)
+            self.h = None
+
+    @property
+    def shape(self):
+        return (self.info.nz, self.info.ny, self.info.nx)
+
+    @property
+    def box(self):
+        return (self.info.pz, self.info.py, self.info.px)
+
+    # ---- device buffers (torch plumbing) ----
+    def _dtype(self):
+        import torch
+        return torch.float32 if self.precision == F32 else torch.float64
+
+    def new_work(self, device="cuda"):
+        import torch
+        return torch.empty(self.info.work_bytes, dtype=torch.uint8, device=device)
+
+    def new_kspec(self, device="cuda"):
+        import torch
+        return torch.empty(self.info.kspec_bytes, dtype=torch.uint8, device=device)
+
+    def gauss_spectrum(self, sigma2, decay, out=None, stream=None):
+        out = self.new_kspec() if out is None else out
+        check(lib.vfd_gauss_spectrum(self.h, float(sigma2), float(decay), _ptr(out), _stream(stream)))
+        return out
+
+    @staticmethod
+    def field_array(fields):
+        """fields: list of dicts with keys in, out, work, kspec (torch tensors),
+        optional base, epi -> ctypes array of vfd_field."""
+        arr = (Field * len(fields))()
+        for i, f in enumerate(fields):
+            arr[i] = Field(_ptr(f["in"]), _ptr(f["out"]), _ptr(f["work"]), _ptr(f["kspec"]),
+                           float(f.get("base", 0.0)), int(f.get("epi", 0)), 0)
+        return arr
+
+    def diffuse(self, fields, stream=None):
+        arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
+        check(lib.vfd_diffuse(self.h, arr, len(arr), _stream(stream)))
+
+    def diffuse_pass(self, fields, pass_index, stream=None):
+        arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
+        check(lib.vfd_diffuse_pass(self.h, arr, len(arr), int(pass_index), _stream(stream)))
+
+    def pass_bytes(self, pass_index):
+        return lib.vfd_plan_pass_bytes(self.h, int(pass_index))
+
+    def mass(self, tensors, stream=None):
+        n = len(tensors)
+        ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in tensors])
+        out = (ctypes.c_double * n)()
+        check(lib.vfd_field_mass(self.h, ptrs, n, out, _stream(stream)))
+        return list(out)
+
+    def r2c(self, real, cplx, stream=None):
+        check(lib.vfd_r2c_3d(self.h, _ptr(real), _ptr(cplx), _stream(stream)))
+
+    def c2r(self, cplx, real, stream=None):
+        check(lib.vfd_c2r_3d(self.h, _ptr(cplx), _ptr(real), _stream(stream)))
+
+
+def length_supported(q):
+    return bool(lib.vfd_length_supported(int(q)))
