@@ -1,0 +1,241 @@
+"""The reference-compatible interface (include/vf_diffusion.h) on the GPU,
+called through the C-ABI with the reference's argument meaning, checked
+against the FP64 oracle (restatement of the same reference functions).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def leg(vfd):
+    from vfd import legacy
+    return legacy
+
+
+def _t(a, dev="cuda"):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _rel(got, ref):
+    got = np.asarray(got, np.float64)
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"  max-abs {np.abs(got - ref).max():.3e} rel-L2 {rel:.3e}", flush=True)
+    return rel
+
+
+@pytest.mark.parametrize("shape,box,K,epi,base", [
+    ((10, 12, 9), (24, 24, 24), 7, 7, 0.5),        # D,H,W data; FD,FH,FW box
+    ((21, 30, 11), (48, 64, 32), 9, 9, 0.125),
+    ((30, 40, 12), (72, 80, 36), 9, 11, 0.0),     # radix-3/5 box lengths
+])
+def test_fftDiffuse3D_constant_vf(vfd, orc, leg, shape, box, K, epi, base):
+    import torch
+    print(f"case fftDiffuse3D shape={shape} box={box}", flush=True)
+    rng = np.random.default_rng(11)
+    u = orc.synth_field(shape, 5, nsrc=4)
+    w = rng.random((K, K, K))
+    w = (w + w[::-1, ::-1, ::-1]) / (2 * w.sum())
+    c = K // 2
+    spec = orc.kernel_spectrum(w, (c, c, c), box)
+    ref = orc.fft_diffuse3d(u, spec, box, orc.PAD_CONSTANT_VF, epi, base)
+    cc = leg.CCtx()
+    cc.DD, cc.DH, cc.DW = shape
+    cc.FFTD, cc.FFTH, cc.FFTW = box
+    cc.KD = cc.KH = cc.KW = K
+    cc.KZ = cc.KY = cc.KX = c
+    d = _t(u)
+    ds = _t(spec.astype(np.complex64))
+    pf, pi = ctypes.c_int(), ctypes.c_int()
+    assert leg.lib.vfdPlan3d(ctypes.byref(pf), *box, 0x2a) == 0
+    assert leg.lib.vfdPlan3d(ctypes.byref(pi), *box, 0x2c) == 0
+    ok = leg.lib.fftDiffuse3D(ctypes.c_void_p(d.data_ptr()), ctypes.c_void_p(ds.data_ptr()), pf, pi, cc,
+                              epi, base, 0, 0)
+    assert ok
+    torch.cuda.synchronize()
+    assert _rel(d.cpu().numpy(), ref) <= 1e-5
+    # a second call reuses the cached workspace and stays identical
+    d2 = _t(u)
+    assert leg.lib.fftDiffuse3D(ctypes.c_void_p(d2.data_ptr()), ctypes.c_void_p(ds.data_ptr()), pf, pi, cc,
+                                epi, base, 1, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(d, d2)
+    assert leg.lib.vfdDestroy(pf) == 0 and leg.lib.vfdDestroy(pi) == 0
+
+
+def test_fftDiffuse3D_rejects_bad_plan(vfd, leg):
+    import torch
+    cc = leg.CCtx()
+    cc.DD = cc.DH = cc.DW = 4
+    cc.FFTD = cc.FFTH = cc.FFTW = 8
+    d = torch.zeros((4, 4, 4), device="cuda")
+    s = torch.zeros((8, 8, 5), dtype=torch.complex64, device="cuda")
+    p = ctypes.c_int()
+    assert leg.lib.vfdPlan3d(ctypes.byref(p), 8, 8, 16, 0x2a) == 0   # wrong box
+    assert not leg.lib.fftDiffuse3D(ctypes.c_void_p(d.data_ptr()), ctypes.c_void_p(s.data_ptr()), p, 0, cc,
+                                    3, 0.0, 0, 0)
+    assert "box" in vfd.last_error()
+    leg.lib.vfdDestroy(p)
+
+
+@pytest.mark.parametrize("quirk", [1, 0])
+def test_computeKernel3D_and_spectrum(vfd, orc, leg, quirk):
+    """computeKernel3D (stencil heat-kernel window) and computeKernelSpectrum3D
+    vs the FP64 restatements, on a small kernel context."""
+    import torch
+    kr, niter, lam, gamma, dt = 5, 2, 0.11, 0.002, 2.0
+    dom = (kr - 1) * 2 + 1
+    kbox = orc.snap_transform_size(dom + 3 - 1)
+    win = 7
+    print(f"case computeKernel3D quirk={quirk} dom={dom} box={kbox}", flush=True)
+    ref_w = orc.compute_kernel3d(kr, lam, gamma, dt, niter, (dom,) * 3, (kbox,) * 3, (win,) * 3, quirk=quirk)
+    kc = leg.CCtx()
+    kc.dt, kc.kernelRadius, kc.niter = dt, kr, niter
+    kc.KD = kc.KH = kc.KW = 3
+    kc.KZ = kc.KY = kc.KX = 1
+    kc.DD = kc.DH = kc.DW = dom
+    kc.FFTD = kc.FFTH = kc.FFTW = kbox
+    kc.windowD = kc.windowH = kc.windowW = win
+    dwin = torch.zeros(win ** 3, dtype=torch.float32, device="cuda")
+    hwin = np.zeros(win ** 3, np.float32)
+    kc.d_data[0] = dwin.data_ptr()
+    kc.h_data[0] = hwin.ctypes.data
+    leg.lib.vfdSetKernelPowerQuirk(quirk)
+    try:
+        assert leg.lib.computeKernel3D(kr, lam, gamma, dt, kc, 0, 0, 0)
+    finally:
+        leg.lib.vfdSetKernelPowerQuirk(1)
+    got = hwin.reshape(win, win, win)
+    assert np.array_equal(got, dwin.cpu().numpy().reshape(win, win, win))
+    assert _rel(got, ref_w) <= 1e-6
+    # spectrum on a chem box
+    box = (24, 20, 16)
+    cc = leg.CCtx()
+    cc.KD = cc.KH = cc.KW = win
+    cc.KZ = cc.KY = cc.KX = win // 2
+    cc.FFTD, cc.FFTH, cc.FFTW = box
+    dspec = torch.zeros((box[0], box[1], box[2] // 2 + 1), dtype=torch.complex64, device="cuda")
+    hspec = np.zeros(dspec.shape, np.complex64)
+    cc.d_kernelspectrum_h[0] = dspec.data_ptr()
+    cc.h_kernelspectrum[0] = hspec.ctypes.data
+    assert leg.lib.computeKernelSpectrum3D(kc, cc, 0)
+    ref_s = orc.kernel_spectrum(got.astype(np.float64), (win // 2,) * 3, box)
+    assert np.abs(hspec - ref_s).max() <= 1e-6 * np.abs(ref_s).max()
+
+
+def test_pad_helpers_bit_exact(vfd, orc, leg):
+    """3D pad / unpad / extract helpers are pure index maps: bit-exact."""
+    import torch
+    rng = np.random.default_rng(2)
+    D, H, W = 5, 6, 7
+    box = (9, 10, 12)
+    u = rng.random((D, H, W)).astype(np.float32)
+    du = _t(u)
+    out = torch.empty(box, device="cuda")
+    leg.lib.padDataConstantVF3D(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(du.data_ptr()), *box, D, H, W,
+                                3, 3, 3, 1, 1, 1, 5, 0.75)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), orc.pad_constant_vf(u, box, 5, 0.75).astype(np.float32))
+    leg.lib.padDataClampToBorder3D(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(du.data_ptr()), *box, D, H, W,
+                                   3, 3, 3, 1, 2, 1)
+    torch.cuda.synchronize()
+    ref = orc.pad_clamp(u.astype(np.float64), box, 1, 2, 1).astype(np.float32)
+    assert np.array_equal(out.cpu().numpy(), ref)
+    k = rng.random((3, 4, 5)).astype(np.float32)
+    kd = _t(k)
+    out.zero_()
+    leg.lib.padKernel3D(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(kd.data_ptr()), *box, 3, 4, 5, 1, 2, 2)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), orc.pad_kernel(k.astype(np.float64), box, 1, 2, 2).astype(np.float32))
+    big = _t(rng.random(box).astype(np.float32))
+    crop = torch.empty((D, H, W), device="cuda")
+    leg.lib.unpadResult3D(ctypes.c_void_p(crop.data_ptr()), ctypes.c_void_p(big.data_ptr()), D, H, W, *box)
+    torch.cuda.synchronize()
+    assert torch.equal(crop, big[:D, :H, :W])
+    win = torch.empty((3, 4, 5), device="cuda")
+    leg.lib.extractCenter3D(ctypes.c_void_p(win.data_ptr()), ctypes.c_void_p(big.data_ptr()), *box, 3, 4, 5)
+    torch.cuda.synchronize()
+    ref = orc.extract_center(big.cpu().numpy().astype(np.float64), (3, 4, 5)).astype(np.float32)
+    assert np.array_equal(win.cpu().numpy(), ref)
+
+
+def test_modulate_and_power(vfd, leg):
+    import torch
+    rng = np.random.default_rng(4)
+    fD, fH, fW = 3, 4, 6
+    n = (fD, fH, fW // 2 + 1)
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    da, db = _t(a), _t(b)
+    leg.lib.modulateAndNormalize3D(ctypes.c_void_p(da.data_ptr()), ctypes.c_void_p(db.data_ptr()), fD, fH, fW, 1)
+    torch.cuda.synchronize()
+    ref = (a.astype(np.complex128) * b) / (fD * fH * fW)
+    assert np.abs(da.cpu().numpy() - ref).max() < 1e-6 * np.abs(ref).max()
+    # complexPower keeps the reference's atan(b/a) phase: exact for Re > 0
+    z = (np.abs(rng.standard_normal(n)) + 0.5 + 0.3j * rng.standard_normal(n)).astype(np.complex64)
+    dz = _t(z)
+    leg.lib.complexPower(ctypes.c_void_p(dz.data_ptr()), fD, fH, fW, 1, 3)
+    torch.cuda.synchronize()
+    ref = z.astype(np.complex128) ** 3
+    assert np.abs(dz.cpu().numpy() - ref).max() < 1e-5 * np.abs(ref).max()
+    # Re < 0: |z|^n cis(n atan(b/a)) = -(z^n) for odd n
+    zn = np.full(n, -2.0 + 0.0j, np.complex64)
+    dzn = _t(zn)
+    leg.lib.complexPower(ctypes.c_void_p(dzn.data_ptr()), fD, fH, fW, 1, 3)
+    torch.cuda.synchronize()
+    assert np.allclose(dzn.cpu().numpy(), 8.0)
+
+
+def test_2d_pads_known_answers(vfd, leg):
+    """2D pad rules (convolutionFFT.cuh:179-363) on a 2x3 field in a 5x6 box."""
+    import torch
+    src = np.array([[1, 2, 3], [4, 5, 6]], np.float32)
+    ds = _t(src)
+    out = torch.empty((5, 6), device="cuda")
+    P = ctypes.c_void_p
+    leg.lib.padDataMirror(P(out.data_ptr()), P(ds.data_ptr()), 5, 6, 2, 3, 3, 3, 1, 1)
+    torch.cuda.synchronize()
+    # rows: y<2 data, y=2 mirror of row 1, y>=3 -> fH-y-1 (1, 0); cols likewise
+    expect = np.array([[1, 2, 3, 3, 2, 1], [4, 5, 6, 6, 2, 1], [4, 5, 6, 6, 2, 1],
+                       [4, 5, 6, 6, 5, 4], [1, 2, 3, 3, 2, 1]], np.float32)
+    got = out.cpu().numpy()
+    # build the expectation from the index rule itself, element by element
+    exp2 = np.empty((5, 6), np.float32)
+    for y in range(5):
+        dy = y if y < 2 else (2 - (y - 2) - 1 if y < 3 else 5 - y - 1)
+        for x in range(6):
+            dx = x if x < 3 else (3 - (x - 3) - 1 if x < 4 else 6 - x - 1)
+            exp2[y, x] = src[dy, dx]
+    assert np.array_equal(got, exp2)
+    assert got[0, 3] == 3 and got[2, 0] == 4 and got[4, 5] == src[0, 0]
+    del expect
+    leg.lib.padDataClampToBorder(P(out.data_ptr()), P(ds.data_ptr()), 5, 6, 2, 3, 3, 3, 1, 1)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert list(got[0]) == [1, 2, 3, 3, 1, 1] and list(got[2]) == [4, 5, 6, 6, 4, 4] and list(got[4]) == [1, 2, 3, 3, 1, 1]
+    leg.lib.padDataConstantVF(P(out.data_ptr()), P(ds.data_ptr()), 5, 6, 2, 3, 3, 3, 1, 1, 3, 9.0)
+    torch.cuda.synchronize()
+    assert np.all(out.cpu().numpy() == 9.0)     # H=2: no row with 0<y<1
+    leg.lib.padDataRightWall(P(out.data_ptr()), P(ds.data_ptr()), 5, 6, 2, 3, 3, 3, 1, 1)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert list(got[0][:3]) == [1, 2, 3] and got[0][3] == 3 and got[0][4] == np.float32(0.0001)
+    assert np.all(got[2][:3] == np.float32(0.0001))   # constant band below the data
+
+
+def test_convolution_clamp_cpu(vfd, leg):
+    """convolutionClampToBorderCPU (host): identity kernel and clamp edges."""
+    data = np.arange(12, dtype=np.float32).reshape(3, 4)
+    k = np.zeros((3, 3), np.float32)
+    k[1, 1] = 1.0
+    res = np.zeros_like(data)
+    leg.lib.convolutionClampToBorderCPU(res.ctypes.data, data.ctypes.data, k.ctypes.data, 3, 4, 3, 3, 1, 1)
+    assert np.array_equal(res, data)
+    k2 = np.zeros((3, 3), np.float32)
+    k2[1, 0] = 1.0   # picks data[y][x+1] with clamp
+    leg.lib.convolutionClampToBorderCPU(res.ctypes.data, data.ctypes.data, k2.ctypes.data, 3, 4, 3, 3, 1, 1)
+    assert list(res[0]) == [1, 2, 3, 3]

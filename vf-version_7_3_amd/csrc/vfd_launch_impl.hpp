@@ -5,11 +5,12 @@
 #include "vfd_launch.h"
 
 // Transform lengths compiled into the library: 2^a 3^b (b <= 2) up to 2304,
-// plus 616 = 8*7*11 (the production kernel-build box).
+// 5 * 2^a up to 1280,
+// plus 616 = 8*7*11 and 308 (the production kernel-build box and its half).
 #define VFD_LENGTHS(X)                                                                         \
     X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(3) X(6) X(12)      \
     X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(9) X(18) X(36) X(72) X(144) X(288) X(576) \
-    X(1152) X(2304) X(616)
+    X(1152) X(2304) X(616) X(308) X(5) X(10) X(20) X(40) X(80) X(160) X(320) X(640) X(1280)
 
 namespace vfd {
 

@@ -1,0 +1,638 @@
+// vfd_legacy.hip -- the reference's diffusion C interface (include/vf_diffusion.h)
+// on top of the five-pass engine.
+//
+// Reference: src/Diffusion/convolutionFFT_common.h:44-424, Diffusion.cu:39-1934,
+// convolutionFFT.cu:22-581, convolutionFFT_gold.cpp:22-58.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/vf_diffusion.h"
+#include "vfd_internal.h"
+
+using namespace vfd;
+
+extern "C" {
+int gpu_id[N_CHEM] = {0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+namespace {
+
+constexpr int kThreads = 256;
+inline unsigned nblocks(long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+// ---------------------------------------------------------------- 3D pads
+// padDataConstantVF3D (convolutionFFT.cuh:590-638)
+__global__ void pad_constant_vf3d(float* dst, const float* src, int fD, int fH, int fW, int D, int H, int W,
+                                  int epi, float base) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)fD * fH * fW;
+    if (i >= n) return;
+    const int x = (int)(i % fW);
+    const int y = (int)((i / fW) % fH);
+    const int z = (int)(i / ((long)fW * fH));
+    const bool in = z < D - 1 && z > 0 && y < H - 1 && y > 0 && x < epi && x > 0;
+    dst[i] = in ? src[((long)z * H + y) * W + x] : base;
+}
+
+// padDataClampToBorder3D (convolutionFFT.cuh:510-585): [0,N) data, [N,N+K) last
+// slice, [N+K,P) first slice on each axis.
+template <typename T>
+__global__ void pad_clamp3d(T* dst, const T* src, int fD, int fH, int fW, int D, int H, int W, int KZ, int KY,
+                            int KX) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)fD * fH * fW;
+    if (i >= n) return;
+    const int x = (int)(i % fW);
+    const int y = (int)((i / fW) % fH);
+    const int z = (int)(i / ((long)fW * fH));
+    const int dz = z < D ? z : (z < D + KZ ? D - 1 : 0);
+    const int dy = y < H ? y : (y < H + KY ? H - 1 : 0);
+    const int dx = x < W ? x : (x < W + KX ? W - 1 : 0);
+    dst[i] = src[((long)dz * H + dy) * W + dx];
+}
+
+// padKernel3D (convolutionFFT.cuh:402-445): centre to the origin, wrap negatives.
+template <typename TD, typename TS>
+__global__ void pad_kernel3d(TD* dst, const TS* src, int fD, int fH, int fW, int kD, int kH, int kW, int KZ,
+                             int KY, int KX) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)kD * kH * kW;
+    if (i >= n) return;
+    const int x = (int)(i % kW);
+    const int y = (int)((i / kW) % kH);
+    const int z = (int)(i / ((long)kW * kH));
+    int kz = z - KZ; if (kz < 0) kz += fD;
+    int ky = y - KY; if (ky < 0) ky += fH;
+    int kx = x - KX; if (kx < 0) kx += fW;
+    dst[((long)kz * fH + ky) * fW + kx] = (TD)src[i];
+}
+
+// unpadResult3D (convolutionFFT.cuh:152-171)
+template <typename T>
+__global__ void unpad3d(T* dst, const T* src, int D, int H, int W, int fH, int fW) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)D * H * W;
+    if (i >= n) return;
+    const int x = (int)(i % W);
+    const int y = (int)((i / W) % H);
+    const int z = (int)(i / ((long)W * H));
+    dst[i] = src[((long)z * fH + y) * fW + x];
+}
+
+// extractCenter3D (convolutionFFT.cuh:77-112)
+template <typename TD, typename TS>
+__global__ void extract_center3d(TD* dst, const TS* src, int D, int H, int W, int wD, int wH, int wW) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)wD * wH * wW;
+    if (i >= n) return;
+    const int x = (int)(i % wW);
+    const int y = (int)((i / wW) % wH);
+    const int z = (int)(i / ((long)wW * wH));
+    const int bD = (D - wD) / 2, bH = (H - wH) / 2, bW = (W - wW) / 2;
+    dst[i] = (TD)src[((long)(z + bD) * H + (y + bH)) * W + (x + bW)];
+}
+
+// modulateAndNormalize_kernel / mulAndScale (convolutionFFT.cuh:690-716):
+// dst[i] = c * (src[i] * dst[i])
+__global__ void modulate_scale(float2* dst, const float2* src, long n, float c) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 a = src[i], b = dst[i];
+    dst[i] = float2{c * (a.x * b.x - a.y * b.y), c * (a.y * b.x + a.x * b.y)};
+}
+
+// complexPower_kernel / cpow (convolutionFFT.cuh:645-684): |z|^n cis(n atan(b/a)).
+__device__ __forceinline__ void pow_quirk(double a, double b, int n, int quirk, double* re, double* im) {
+    const double ab2 = a * a + b * b;
+    if (ab2 == 0.0) {  // 0^n (the reference's atan(0/0) yields NaN here)
+        *re = n == 0 ? 1.0 : 0.0;
+        *im = 0.0;
+        return;
+    }
+    const double ph = quirk ? atan(b / a) : atan2(b, a);
+    const double mag = pow(ab2, 0.5 * n);
+    *re = mag * cos(n * ph);
+    *im = mag * sin(n * ph);
+}
+
+__global__ void complex_power(float2* d, long n, int p) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // FP32 in, FP32 out, evaluated like cpow: powf(|z|^2, n/2) cis(n atanf(b/a))
+    const float a = d[i].x, b = d[i].y;
+    const float ab2 = a * a + b * b;
+    const float nf = (float)p;
+    const float ang = nf * atanf(b / a);
+    const float mag = powf(ab2, nf / 2.0f);
+    d[i] = float2{mag * cosf(ang), mag * sinf(ang)};
+}
+
+// Stencil spectrum power for the kernel build: data_spec *= P(k) * scale with
+// P = pow(S^(k), n) and S^ the DFT of the 7-point stencil wrapped to the origin,
+// S^ = 1 - 6 lambda - gamma dt + 2 lambda (cos tz + cos ty + cos tx).
+__global__ void stencil_power_multiply(double2* spec, int fD, int fH, int fW, int hx, double lambda,
+                                       double c0, int n, int quirk, double scale) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)fD * fH * hx;
+    if (i >= total) return;
+    const int kx = (int)(i % hx);
+    const int ky = (int)((i / hx) % fH);
+    const int kz = (int)(i / ((long)hx * fH));
+    const double tp = 6.283185307179586476925286766559005768;
+    const double s = c0 + 2.0 * lambda * (cos(tp * kz / fD) + cos(tp * ky / fH) + cos(tp * kx / fW));
+    double pr, pi;
+    pow_quirk(s, 0.0, n, quirk, &pr, &pi);
+    const double2 v = spec[i];
+    spec[i] = double2{scale * (v.x * pr - v.y * pi), scale * (v.x * pi + v.y * pr)};
+}
+
+__global__ void to_fcomplex(float2* dst, const double2* src, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = float2{(float)src[i].x, (float)src[i].y};
+}
+
+template <typename T>
+__global__ void fill(T* d, long n, T v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
+// ---------------------------------------------------------------- 2D pads
+// padKernel_kernel (convolutionFFT.cuh:52-75)
+__global__ void pad_kernel2d(float* dst, const float* src, int fH, int fW, int kH, int kW, int KY, int KX) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)kH * kW) return;
+    const int x = (int)(i % kW), y = (int)(i / kW);
+    int ky = y - KY; if (ky < 0) ky += fH;
+    int kx = x - KX; if (kx < 0) kx += fW;
+    dst[(long)ky * fW + kx] = src[i];
+}
+
+// padDataConstantVF_kernel (convolutionFFT.cuh:179-218)
+__global__ void pad_constant_vf2d(float* dst, const float* src, int fH, int fW, int H, int W, int epi,
+                                  float base) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)fH * fW) return;
+    const int x = (int)(i % fW), y = (int)(i / fW);
+    const bool in = y < H - 1 && y > 0 && x < epi && x > 0;
+    dst[i] = in ? src[(long)y * W + x] : base;
+}
+
+// padDataRightWall_kernel (convolutionFFT.cuh:223-310): data, a constant band
+// (0.0001) below the data, mirror to the right, mirror-of-fft below-right.
+__global__ void pad_right_wall2d(float* dst, const float* src, int fH, int fW, int H, int W, int KY, int KX) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)fH * fW) return;
+    const int x = (int)(i % fW), y = (int)(i / fW);
+    const int bH = H + KY, bW = W + KX;
+    int dy = -1, dx = -1;
+    do {
+        if (y < H) dy = y;
+        if (x < W) {
+            dx = x;
+            if (y >= H && y < bH) { dy = -1; break; }
+        }
+        if (y >= H && y < bH) dy = H - (y - H) - 1;
+        if (x >= W && x < bW) {
+            dx = W - (x - W) - 1;
+            if (y >= bH) { dy = fH - y - 1; break; }
+        }
+        if (y >= bH) dy = -1;
+        if (x >= bW) dx = -1;
+    } while (0);
+    dst[i] = (dx < 0 || dy < 0) ? 0.0001f : src[(long)dy * W + dx];
+}
+
+// padDataMirror_kernel (convolutionFFT.cuh:315-363)
+__global__ void pad_mirror2d(float* dst, const float* src, int fH, int fW, int H, int W, int KY, int KX) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)fH * fW) return;
+    const int x = (int)(i % fW), y = (int)(i / fW);
+    const int bH = H + KY, bW = W + KX;
+    int dy = 0, dx = 0;
+    if (y < H) dy = y;
+    if (x < W) dx = x;
+    if (y >= H && y < bH) dy = H - (y - H) - 1;
+    if (x >= W && x < bW) dx = W - (x - W) - 1;
+    if (y >= bH) dy = fH - y - 1;
+    if (x >= bW) dx = fW - x - 1;
+    dst[i] = src[(long)dy * W + dx];
+}
+
+// ------------------------------------------------------------------ state
+std::mutex g_mu;  // serializes fftDiffuse3D / kernel precompute (shared workspace)
+int g_quirk = 1;
+
+struct PlanRec {
+    int nz, ny, nx, type;
+};
+std::map<int, PlanRec> g_plans;
+int g_next_plan = 1;
+
+struct WsKey {
+    int dev, D, H, W, FD, FH, FW;
+    bool operator<(const WsKey& o) const {
+        return std::tie(dev, D, H, W, FD, FH, FW) < std::tie(o.dev, o.D, o.H, o.W, o.FD, o.FH, o.FW);
+    }
+};
+struct Ws {
+    Plan plan;
+    void* work = nullptr;
+};
+std::map<WsKey, Ws> g_ws;
+
+bool fail(int rc) {
+    (void)rc;
+    fprintf(stderr, "vfd: %s\n", vfd_last_error());
+    return false;
+}
+
+bool hip_ok(hipError_t e, const char* what) { return hip_check(e, what) == VFD_OK || fail(-1); }
+
+}  // namespace
+
+// ====================================================================== C-ABI
+extern "C" {
+
+int snapTransformSize(int dataSize) {
+    // Diffusion.cu:39-70
+    if (dataSize % 16) dataSize = dataSize - dataSize % 16 + 16;
+    int hiBit;
+    for (hiBit = 31; hiBit >= 0; hiBit--)
+        if ((unsigned)dataSize & (1U << hiBit)) break;
+    const unsigned lowPOT = 1U << hiBit;
+    if (lowPOT == (unsigned)dataSize) return dataSize;
+    const unsigned hiPOT = 1U << (hiBit + 1);
+    if (hiPOT <= 1024) return (int)hiPOT;
+    return (int)pow(2.0, ceil(log((double)dataSize) / log(2.0)));
+}
+
+void vfdSetKernelPowerQuirk(int on) { g_quirk = on ? 1 : 0; }
+
+int vfdPlan3d(cufftHandle* plan, int nz, int ny, int nx, int type) {
+    if (!plan || nz < 1 || ny < 1 || nx < 1) return set_error(VFD_ERR_INVALID, "vfdPlan3d: bad arguments");
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int id = g_next_plan++;
+    g_plans[id] = PlanRec{nz, ny, nx, type};
+    *plan = id;
+    return VFD_OK;
+}
+
+int vfdDestroy(cufftHandle plan) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_plans.erase(plan) ? VFD_OK : set_error(VFD_ERR_NOTFOUND, "vfdDestroy: unknown plan %d", plan);
+}
+
+// H2D / D2H (Diffusion.cu:72-122; under M40 the spectrum stays on the device)
+void H2D(int ic, c_ctx* cc, int np) {
+    if (!cc || ic < 0 || ic >= N_CHEM || np < 0) {
+        set_error(VFD_ERR_INVALID, "H2D: bad arguments");
+        fail(-1);
+        return;
+    }
+    if (!hip_ok(hipSetDevice(cc->dev_id[ic]), "H2D hipSetDevice")) return;
+    hip_ok(hipMemcpy(cc->d_data[ic], cc->h_ibuffs[ic], (size_t)np * sizeof(float), hipMemcpyHostToDevice), "H2D");
+}
+
+void D2H(int ic, c_ctx* cc, int np) {
+    if (!cc || ic < 0 || ic >= N_CHEM || np < 0) {
+        set_error(VFD_ERR_INVALID, "D2H: bad arguments");
+        fail(-1);
+        return;
+    }
+    if (!hip_ok(hipSetDevice(cc->dev_id[ic]), "D2H hipSetDevice")) return;
+    hip_ok(hipMemcpy(cc->h_obuffs[ic], cc->d_data[ic], (size_t)np * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+}
+
+void reportMemUsageGPU(void) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        printf("GPU memory usage: unavailable\n");
+        return;
+    }
+    const double gb = 1024.0 * 1024.0 * 1024.0;
+    printf("GPU memory usage: used = %f GB, free = %f GB, total = %f GB\n\n", (double)(tot - fr) / gb,
+           (double)fr / gb, (double)tot / gb);
+}
+
+// --------------------------------------------------------- the hot path
+bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlanFwd, cufftHandle fftPlanInv,
+                  c_ctx cctx, int epiBoundary, float baseChem, int chemIndex, int iter) {
+    (void)chemIndex;
+    (void)iter;
+    if (!d_Data || !d_KernelSpectrum) return fail(set_error(VFD_ERR_INVALID, "fftDiffuse3D: null buffer"));
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (cufftHandle h : {fftPlanFwd, fftPlanInv}) {
+        if (h == 0) continue;  // 0: no plan supplied, the box comes from cctx
+        auto it = g_plans.find(h);
+        if (it == g_plans.end()) return fail(set_error(VFD_ERR_NOTFOUND, "fftDiffuse3D: unknown plan %d", h));
+        const PlanRec& r = it->second;
+        if (r.nz != cctx.FFTD || r.ny != cctx.FFTH || r.nx != cctx.FFTW)
+            return fail(set_error(VFD_ERR_INVALID, "fftDiffuse3D: plan %d box %dx%dx%d != cctx FFT box %dx%dx%d", h,
+                                  r.nz, r.ny, r.nx, cctx.FFTD, cctx.FFTH, cctx.FFTW));
+    }
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+    const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
+    auto it = g_ws.find(key);
+    if (it == g_ws.end()) {
+        vfd_plan_desc d{cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW, VFD_F32, VFD_PAD_CONSTANT_VF,
+                        VFD_KSPEC_COMPLEX_STD};
+        Ws ws;
+        if (int rc = plan_init(&d, &ws.plan)) return fail(rc);
+        if (!hip_ok(hipMalloc(&ws.work, ws.plan.work_bytes), "fftDiffuse3D workspace")) return false;
+        it = g_ws.emplace(key, ws).first;
+    }
+    vfd_field f{d_Data, d_Data, it->second.work, d_KernelSpectrum, (double)baseChem, epiBoundary, 0};
+    if (int rc = plan_diffuse(it->second.plan, &f, 1, nullptr)) return fail(rc);
+    return hip_ok(hipStreamSynchronize(nullptr), "fftDiffuse3D");
+}
+
+// --------------------------------------------------------- kernel build
+// computeKernel3D (Diffusion.cu:1078-1531), FP64 on the device: the 7-point
+// stencil at the centre of the kernel domain, niter rounds of clamp-to-border
+// pad -> R2C -> x S^(kernelRadius-1-[first]) / N -> C2R -> unpad, then the
+// centred window.  S^ is evaluated in closed form (the exact DFT of the wrapped
+// stencil) instead of an R2C of the padded stencil.
+bool computeKernel3D(int kernelRadius, float lambda, float gamma, float dt, c_ctx cctx, cufftHandle fftPlanFwd,
+                     cufftHandle fftPlanInv, short int ic) {
+    (void)fftPlanFwd;
+    (void)fftPlanInv;
+    if (ic < 0 || ic >= N_CHEM) return fail(set_error(VFD_ERR_INVALID, "computeKernel3D: bad chem index"));
+    const int D = cctx.DD, H = cctx.DH, W = cctx.DW;
+    const int FD = cctx.FFTD, FH = cctx.FFTH, FW = cctx.FFTW;
+    const int wD = cctx.windowD, wH = cctx.windowH, wW = cctx.windowW;
+    const int KD = cctx.KD, KH = cctx.KH, KW = cctx.KW;
+    if (D < 3 || H < 3 || W < 3 || FD < D || FH < H || FW < W || wD > D || wH > H || wW > W || cctx.niter < 1)
+        return fail(set_error(VFD_ERR_INVALID, "computeKernel3D: bad kernel context dimensions"));
+    if (!cctx.d_data[ic]) return fail(set_error(VFD_ERR_INVALID, "computeKernel3D: d_data[%d] is null", ic));
+    std::lock_guard<std::mutex> lk(g_mu);
+    vfd_plan_desc d{FD, FH, FW, FD, FH, FW, VFD_F64, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    Plan plan;
+    if (int rc = plan_init(&d, &plan)) return fail(rc);
+    const long dn = (long)D * H * W, fn = (long)FD * FH * FW, sn = (long)FD * FH * plan.hx;
+    // host: stencil in the middle block of the domain (Diffusion.cu:1187-1229)
+    std::vector<double> h_data(dn, 0.0);
+    const double lam = lambda, c0 = 1.0 - 6.0 * (double)lambda - (double)gamma * (double)dt;
+    double hk[27] = {0};
+    hk[0 * 9 + 1 * 3 + 1] = lam; hk[2 * 9 + 1 * 3 + 1] = lam;
+    hk[1 * 9 + 0 * 3 + 1] = lam; hk[1 * 9 + 2 * 3 + 1] = lam;
+    hk[1 * 9 + 1 * 3 + 0] = lam; hk[1 * 9 + 1 * 3 + 2] = lam;
+    hk[1 * 9 + 1 * 3 + 1] = c0;
+    (void)KD; (void)KH; (void)KW;
+    const int sk = D / 2 - 1, si = H / 2 - 1, sj = W / 2 - 1;
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) h_data[((long)(sk + k) * H + (si + i)) * W + (sj + j)] = hk[(k * 3 + i) * 3 + j];
+    double *d_data = nullptr, *d_pad = nullptr;
+    double2* d_spec = nullptr;
+    bool ok = hip_ok(hipMalloc(&d_data, dn * sizeof(double)), "kernel build alloc") &&
+              hip_ok(hipMalloc(&d_pad, fn * sizeof(double)), "kernel build alloc") &&
+              hip_ok(hipMalloc(&d_spec, sn * sizeof(double2)), "kernel build alloc") &&
+              hip_ok(hipMemcpy(d_data, h_data.data(), dn * sizeof(double), hipMemcpyHostToDevice), "kernel upload");
+    const double scale = 1.0 / ((double)FD * FH * FW);
+    for (int it = 0; ok && it < cctx.niter; ++it) {
+        hipLaunchKernelGGL(pad_clamp3d<double>, dim3(nblocks(fn)), dim3(kThreads), 0, nullptr, d_pad, d_data, FD, FH,
+                           FW, D, H, W, 1, 1, 1);
+        ok = plan_r2c(plan, d_pad, d_spec, nullptr) == VFD_OK || fail(-1);
+        if (!ok) break;
+        const int n = kernelRadius - (it == 0 ? 1 : 0) - 1;
+        hipLaunchKernelGGL(stencil_power_multiply, dim3(nblocks(sn)), dim3(kThreads), 0, nullptr, d_spec, FD, FH, FW,
+                           plan.hx, lam, c0, n, g_quirk, scale);
+        ok = plan_c2r(plan, d_spec, d_pad, nullptr) == VFD_OK || fail(-1);
+        if (!ok) break;
+        hipLaunchKernelGGL(unpad3d<double>, dim3(nblocks(dn)), dim3(kThreads), 0, nullptr, d_data, d_pad, D, H, W,
+                           FH, FW);
+        ok = hip_ok(hipGetLastError(), "kernel build launch");
+    }
+    const long wn = (long)wD * wH * wW;
+    if (ok) {
+        hipLaunchKernelGGL((extract_center3d<float, double>), dim3(nblocks(wn)), dim3(kThreads), 0, nullptr,
+                           cctx.d_data[ic], d_data, D, H, W, wD, wH, wW);
+        ok = hip_ok(hipGetLastError(), "extract window") && hip_ok(hipDeviceSynchronize(), "kernel build");
+    }
+    if (ok && cctx.h_data[ic])
+        ok = hip_ok(hipMemcpy(cctx.h_data[ic], cctx.d_data[ic], wn * sizeof(float), hipMemcpyDeviceToHost),
+                    "window download");
+    (void)hipFree(d_data);
+    (void)hipFree(d_pad);
+    (void)hipFree(d_spec);
+    return ok;
+}
+
+// computeKernelSpectrum3D (Diffusion.cu:1533-1619): window (kernel_cctx.d_data[ic],
+// chem KD x KH x KW, centre KZ,KY,KX) padded into the chem FFT box, R2C in FP64,
+// stored as fComplex in chem_cctx.d_kernelspectrum_h[ic] (and h_kernelspectrum).
+bool computeKernelSpectrum3D(c_ctx kernel_cctx, c_ctx chem_cctx, short int ic) {
+    if (ic < 0 || ic >= N_CHEM) return fail(set_error(VFD_ERR_INVALID, "computeKernelSpectrum3D: bad chem index"));
+    const int FD = chem_cctx.FFTD, FH = chem_cctx.FFTH, FW = chem_cctx.FFTW;
+    const int KD = chem_cctx.KD, KH = chem_cctx.KH, KW = chem_cctx.KW;
+    const float* d_win = kernel_cctx.d_data[ic];
+    fComplex* d_out = chem_cctx.d_kernelspectrum_h[ic];
+    if (!d_win || !d_out) return fail(set_error(VFD_ERR_INVALID, "computeKernelSpectrum3D: null buffer"));
+    if (KD > FD || KH > FH || KW > FW)
+        return fail(set_error(VFD_ERR_INVALID, "computeKernelSpectrum3D: window larger than the FFT box"));
+    std::lock_guard<std::mutex> lk(g_mu);
+    vfd_plan_desc d{FD, FH, FW, FD, FH, FW, VFD_F64, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    Plan plan;
+    if (int rc = plan_init(&d, &plan)) return fail(rc);
+    const long fn = (long)FD * FH * FW, sn = (long)FD * FH * plan.hx, kn = (long)KD * KH * KW;
+    double* d_pad = nullptr;
+    double2* d_spec = nullptr;
+    bool ok = hip_ok(hipMalloc(&d_pad, fn * sizeof(double)), "spectrum alloc") &&
+              hip_ok(hipMalloc(&d_spec, sn * sizeof(double2)), "spectrum alloc");
+    if (ok) {
+        hipLaunchKernelGGL(fill<double>, dim3(nblocks(fn)), dim3(kThreads), 0, nullptr, d_pad, fn, 0.0);
+        hipLaunchKernelGGL((pad_kernel3d<double, float>), dim3(nblocks(kn)), dim3(kThreads), 0, nullptr, d_pad,
+                           d_win, FD, FH, FW, KD, KH, KW, chem_cctx.KZ, chem_cctx.KY, chem_cctx.KX);
+        ok = hip_ok(hipGetLastError(), "pad kernel") && (plan_r2c(plan, d_pad, d_spec, nullptr) == VFD_OK || fail(-1));
+    }
+    if (ok) {
+        hipLaunchKernelGGL(to_fcomplex, dim3(nblocks(sn)), dim3(kThreads), 0, nullptr, (float2*)d_out, d_spec, sn);
+        ok = hip_ok(hipGetLastError(), "spectrum convert") && hip_ok(hipDeviceSynchronize(), "spectrum");
+    }
+    if (ok && chem_cctx.h_kernelspectrum[ic])
+        ok = hip_ok(hipMemcpy(chem_cctx.h_kernelspectrum[ic], d_out, sn * sizeof(fComplex), hipMemcpyDeviceToHost),
+                    "spectrum download");
+    (void)hipFree(d_pad);
+    (void)hipFree(d_spec);
+    return ok;
+}
+
+// computeKernel3DBatch (Diffusion.cu:988-1076)
+bool computeKernel3DBatch(int kernelRadius, float lambda[N_CHEM], float gamma[N_CHEM], float dt, c_ctx kern_cctx,
+                          c_ctx chem_cctx) {
+    bool ok = true;
+    for (int ic = 0; ic < N_CHEM && ok; ++ic) {
+        if (!hip_ok(hipSetDevice(chem_cctx.dev_id[ic]), "hipSetDevice")) return false;
+        ok = computeKernel3D(kernelRadius, lambda[ic], gamma[ic], dt, kern_cctx, 0, 0, (short)ic) &&
+             computeKernelSpectrum3D(kern_cctx, chem_cctx, (short)ic);
+    }
+    return ok;
+}
+
+// ------------------------------------------------------------ 3D helpers
+void extractCenter3D(float* d_Dst, float* d_Src, int dataD, int dataH, int dataW, int windowD, int windowH,
+                     int windowW) {
+    const long n = (long)windowD * windowH * windowW;
+    hipLaunchKernelGGL((extract_center3d<float, float>), dim3(nblocks(n)), dim3(kThreads), 0, nullptr, d_Dst, d_Src,
+                       dataD, dataH, dataW, windowD, windowH, windowW);
+    hip_ok(hipGetLastError(), "extractCenter3D");
+}
+
+void unpadResult3D(float* d_Dst, float* d_Src, int dataD, int dataH, int dataW, int fftD, int fftH, int fftW) {
+    (void)fftD;
+    const long n = (long)dataD * dataH * dataW;
+    hipLaunchKernelGGL(unpad3d<float>, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, d_Dst, d_Src, dataD, dataH,
+                       dataW, fftH, fftW);
+    hip_ok(hipGetLastError(), "unpadResult3D");
+}
+
+void padKernel3D(float* d_Dst, float* d_Src, int fftD, int fftH, int fftW, int kernelD, int kernelH, int kernelW,
+                 int kernelZ, int kernelY, int kernelX) {
+    const long n = (long)kernelD * kernelH * kernelW;
+    hipLaunchKernelGGL((pad_kernel3d<float, float>), dim3(nblocks(n)), dim3(kThreads), 0, nullptr, d_Dst, d_Src,
+                       fftD, fftH, fftW, kernelD, kernelH, kernelW, kernelZ, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padKernel3D");
+}
+
+void padDataClampToBorder3D(float* d_PaddedData, float* d_Data, int fftD, int fftH, int fftW, int dataD, int dataH,
+                            int dataW, int kernelD, int kernelH, int kernelW, int kernelZ, int kernelY, int kernelX) {
+    (void)kernelD; (void)kernelH; (void)kernelW;
+    const long n = (long)fftD * fftH * fftW;
+    hipLaunchKernelGGL(pad_clamp3d<float>, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, d_PaddedData, d_Data, fftD,
+                       fftH, fftW, dataD, dataH, dataW, kernelZ, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padDataClampToBorder3D");
+}
+
+void padDataConstantVF3D(float* d_Dst, float* d_Src, int fftD, int fftH, int fftW, int dataD, int dataH, int dataW,
+                         int kernelD, int kernelH, int kernelW, int kernelZ, int kernelY, int kernelX,
+                         int epiBoundary, float baseChem) {
+    (void)kernelD; (void)kernelH; (void)kernelW; (void)kernelZ; (void)kernelY; (void)kernelX;
+    const long n = (long)fftD * fftH * fftW;
+    hipLaunchKernelGGL(pad_constant_vf3d, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, d_Dst, d_Src, fftD, fftH,
+                       fftW, dataD, dataH, dataW, epiBoundary, baseChem);
+    hip_ok(hipGetLastError(), "padDataConstantVF3D");
+}
+
+void modulateAndNormalize3D(fComplex* d_Dst, fComplex* d_Src, int fftD, int fftH, int fftW, int padding) {
+    const long n = (long)fftD * fftH * (fftW / 2 + padding);
+    hipLaunchKernelGGL(modulate_scale, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, (float2*)d_Dst,
+                       (const float2*)d_Src, n, 1.0f / (float)(fftW * fftH * fftD));
+    hip_ok(hipGetLastError(), "modulateAndNormalize3D");
+}
+
+void modulate3D(fComplex* d_Dst, fComplex* d_Src, int fftD, int fftH, int fftW, int padding) {
+    const long n = (long)fftD * fftH * (fftW / 2 + padding);
+    hipLaunchKernelGGL(modulate_scale, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, (float2*)d_Dst,
+                       (const float2*)d_Src, n, 1.0f);
+    hip_ok(hipGetLastError(), "modulate3D");
+}
+
+void complexPower(fComplex* d_Dst, int fftD, int fftH, int fftW, int padding, int n) {
+    const long cnt = (long)fftD * fftH * (fftW / 2 + padding);
+    hipLaunchKernelGGL(complex_power, dim3(nblocks(cnt)), dim3(kThreads), 0, nullptr, (float2*)d_Dst, cnt, n);
+    hip_ok(hipGetLastError(), "complexPower");
+}
+
+// ------------------------------------------------------------ 2D helpers
+void extractCenter(float* d_Dst, float* d_Src, int dataH, int dataW, int windowH, int windowW) {
+    extractCenter3D(d_Dst, d_Src, 1, dataH, dataW, 1, windowH, windowW);
+}
+
+void unpadResult(float* d_Dst, float* d_Src, int dataH, int dataW, int fftH, int fftW) {
+    unpadResult3D(d_Dst, d_Src, 1, dataH, dataW, 1, fftH, fftW);
+}
+
+void padKernel(float* d_PaddedKernel, float* d_Kernel, int fftH, int fftW, int kernelH, int kernelW, int kernelY,
+               int kernelX) {
+    hipLaunchKernelGGL(pad_kernel2d, dim3(nblocks((long)kernelH * kernelW)), dim3(kThreads), 0, nullptr,
+                       d_PaddedKernel, d_Kernel, fftH, fftW, kernelH, kernelW, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padKernel");
+}
+
+void padDataConstantVF(float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW, int kernelH,
+                       int kernelW, int kernelY, int kernelX, int epiBoundary, float baseChem) {
+    (void)kernelH; (void)kernelW; (void)kernelY; (void)kernelX;
+    hipLaunchKernelGGL(pad_constant_vf2d, dim3(nblocks((long)fftH * fftW)), dim3(kThreads), 0, nullptr, d_PaddedData,
+                       d_Data, fftH, fftW, dataH, dataW, epiBoundary, baseChem);
+    hip_ok(hipGetLastError(), "padDataConstantVF");
+}
+
+void padDataRightWall(float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW, int kernelH,
+                      int kernelW, int kernelY, int kernelX) {
+    (void)kernelH; (void)kernelW;
+    hipLaunchKernelGGL(pad_right_wall2d, dim3(nblocks((long)fftH * fftW)), dim3(kThreads), 0, nullptr, d_PaddedData,
+                       d_Data, fftH, fftW, dataH, dataW, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padDataRightWall");
+}
+
+void padDataMirror(float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW, int kernelH,
+                   int kernelW, int kernelY, int kernelX) {
+    (void)kernelH; (void)kernelW;
+    hipLaunchKernelGGL(pad_mirror2d, dim3(nblocks((long)fftH * fftW)), dim3(kThreads), 0, nullptr, d_PaddedData,
+                       d_Data, fftH, fftW, dataH, dataW, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padDataMirror");
+}
+
+void padDataClampToBorder(float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW, int kernelH,
+                          int kernelW, int kernelY, int kernelX) {
+    (void)kernelH; (void)kernelW;
+    hipLaunchKernelGGL(pad_clamp3d<float>, dim3(nblocks((long)fftH * fftW)), dim3(kThreads), 0, nullptr,
+                       d_PaddedData, d_Data, 1, fftH, fftW, 1, dataH, dataW, 0, kernelY, kernelX);
+    hip_ok(hipGetLastError(), "padDataClampToBorder");
+}
+
+void padData2D(pad_t pt, float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW, int kernelH,
+               int kernelW, int kernelY, int kernelX, int epiBoundary, float baseChem) {
+    switch (pt) {
+        case pClampToBorder:
+            padDataClampToBorder(d_PaddedData, d_Data, fftH, fftW, dataH, dataW, kernelH, kernelW, kernelY, kernelX);
+            break;
+        case pRightWall:
+            padDataRightWall(d_PaddedData, d_Data, fftH, fftW, dataH, dataW, kernelH, kernelW, kernelY, kernelX);
+            break;
+        case pMirror:
+            padDataMirror(d_PaddedData, d_Data, fftH, fftW, dataH, dataW, kernelH, kernelW, kernelY, kernelX);
+            break;
+        case pConstantVF:
+            padDataConstantVF(d_PaddedData, d_Data, fftH, fftW, dataH, dataW, kernelH, kernelW, kernelY, kernelX,
+                              epiBoundary, baseChem);
+            break;
+    }
+}
+
+void modulateAndNormalize(fComplex* d_Dst, fComplex* d_Src, int fftH, int fftW, int padding) {
+    const long n = (long)fftH * (fftW / 2 + padding);
+    hipLaunchKernelGGL(modulate_scale, dim3(nblocks(n)), dim3(kThreads), 0, nullptr, (float2*)d_Dst,
+                       (const float2*)d_Src, n, 1.0f / (float)(fftW * fftH));
+    hip_ok(hipGetLastError(), "modulateAndNormalize");
+}
+
+// convolutionFFT_gold.cpp:22-58: FP64-accumulated direct 2D convolution with
+// clamp-to-border addressing.
+void convolutionClampToBorderCPU(float* h_Result, float* h_Data, float* h_Kernel, int dataH, int dataW, int kernelH,
+                                 int kernelW, int kernelY, int kernelX) {
+    for (int y = 0; y < dataH; y++)
+        for (int x = 0; x < dataW; x++) {
+            double sum = 0;
+            for (int ky = -(kernelH - kernelY - 1); ky <= kernelY; ky++)
+                for (int kx = -(kernelW - kernelX - 1); kx <= kernelX; kx++) {
+                    int dy = y + ky, dx = x + kx;
+                    dy = dy < 0 ? 0 : (dy >= dataH ? dataH - 1 : dy);
+                    dx = dx < 0 ? 0 : (dx >= dataW ? dataW - 1 : dx);
+                    sum += (double)h_Data[dy * dataW + dx] * (double)h_Kernel[(kernelY - ky) * kernelW + (kernelX - kx)];
+                }
+            h_Result[y * dataW + x] = (float)sum;
+        }
+}
+
+}  // extern "C"

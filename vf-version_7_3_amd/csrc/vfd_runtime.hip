@@ -444,7 +444,8 @@ bool length_supported(int q) {
         VFD_CASE(256) VFD_CASE(512) VFD_CASE(1024) VFD_CASE(2048) VFD_CASE(3) VFD_CASE(6) VFD_CASE(12)
         VFD_CASE(24) VFD_CASE(48) VFD_CASE(96) VFD_CASE(192) VFD_CASE(384) VFD_CASE(768) VFD_CASE(1536)
         VFD_CASE(9) VFD_CASE(18) VFD_CASE(36) VFD_CASE(72) VFD_CASE(144) VFD_CASE(288) VFD_CASE(576)
-        VFD_CASE(1152) VFD_CASE(2304) VFD_CASE(616)
+        VFD_CASE(1152) VFD_CASE(2304) VFD_CASE(616) VFD_CASE(308) VFD_CASE(5) VFD_CASE(10) VFD_CASE(20)
+        VFD_CASE(40) VFD_CASE(80) VFD_CASE(160) VFD_CASE(320) VFD_CASE(640) VFD_CASE(1280)
 #undef VFD_CASE
         return true;
         default:
