@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--fields", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
     return ap.parse_args()
 
 
@@ -102,6 +104,8 @@ def main():
     arr = plan.field_array([{"in": f, "out": f, "work": w, "kspec": k}
                             for f, w, k in zip(fields, works, kspecs)])
     torch.cuda.synchronize()
+    # schedule autotuner (per pass: radix 16/8 and work-group size), untimed
+    schedule = plan.schedule if args.no_autotune else plan.autotune(len(fields), stream)[0]
 
     for _ in range(args.warmup):
         plan.diffuse(arr, stream)
@@ -152,7 +156,8 @@ def main():
         "config": {"workload": f"{nf} FP32 cytokine fields, {nx}x{ny}x{nz} patch grid, zero-padded "
                                f"linear convolution on {2*nx}x{2*ny}x{2*nz} FFT box, Gaussian "
                                "Green's kernel x decay, 1 tick per step",
-                   "fields": nf, "grid": [nx, ny, nz], "fft_box": [2 * nx, 2 * ny, 2 * nz]},
+                   "fields": nf, "grid": [nx, ny, nz], "fft_box": [2 * nx, 2 * ny, 2 * nz],
+                   "schedule": dict(zip(vfd.PASS_NAMES, schedule))},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "roofline": {
             "bound": "hbm",

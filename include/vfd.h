@@ -101,6 +101,15 @@ enum { VFD_PASS_XFWD = 0, VFD_PASS_YFWD = 1, VFD_PASS_ZCONV = 2, VFD_PASS_YINV =
 int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream);
 double vfd_plan_pass_bytes(const vfd_plan* plan, int pass);
 
+/* Schedule autotuner: every compiled schedule variant (radix 16/8, work-group
+ * size) of every pass is timed on scratch buffers for nfields fields; the
+ * fastest per pass is kept in the plan and remembered for plans of the same
+ * shape created later in the process.  best_ms (optional, 5 floats) receives
+ * the winning per-pass times.  Allocates scratch; synchronizes stream. */
+int vfd_plan_autotune(vfd_plan* plan, int nfields, void* stream, float* best_ms);
+int vfd_plan_get_schedule(const vfd_plan* plan, int* variants);  /* 5 ints */
+int vfd_plan_set_schedule(vfd_plan* plan, const int* variants);  /* 5 ints, 0..3 */
+
 /* Real kernel spectrum (VFD_KSPEC_REAL) of the normalized sampled Gaussian
  * g(j) ~ exp(-j^2/(2 sigma2)), |j| <= ceil(6 sqrt(sigma2)) per axis, times decay,
  * computed in FP64 and stored in the plan's precision. */

@@ -1,11 +1,12 @@
 // vfd_fft.hpp -- device-side building blocks of the CDNA4 FFT passes.
 //
 // A batch of length-Q complex DFTs is computed per work-group by a mixed-radix
-// Stockham autosort (radices 2,3,4,5,7,8,11,13).  Each thread owns a fixed set
-// of butterflies per stage; data moves between stages through LDS, so only the
-// first stage's inputs and the last stage's outputs touch registers that the
-// surrounding pass kernel fills from / drains to HBM.  Twiddles come from a
-// per-length table W_P^k (P = Q * split) computed in FP64 on the host.
+// Stockham autosort (radices 2,3,4,5,7,8,11,13,16).  Each thread owns a fixed
+// set of butterflies per stage; data moves between stages through LDS, so only
+// the first stage's inputs and the last stage's outputs live in registers that
+// the surrounding pass kernel fills from / drains to HBM.  Twiddles are read
+// from a per-length table W_P^k (P = Q * split) computed in FP64 on the host
+// and staged in LDS by the pass kernel.
 //
 // Convention: forward = exp(-2 pi i nk/Q), inverse = exp(+2 pi i nk/Q), both
 // unnormalized (the reference's cuFFT plans, src/Diffusion/Diffusion.cu:1814,1842).
@@ -91,24 +92,72 @@ template <bool INV, typename T> struct Bfly<4, INV, T> {
     }
 };
 
+// multiply by W8^1 = h(1 - i) (fwd) / h(1 + i) (inv)
+template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_1(cx<T> c) {
+    const T h = T(0.70710678118654752440084436210484903928);
+    return INV ? cx<T>{h * (c.x - c.y), h * (c.x + c.y)} : cx<T>{h * (c.x + c.y), h * (c.y - c.x)};
+}
+// multiply by W8^3 = h(-1 - i) (fwd) / h(-1 + i) (inv)
+template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_3(cx<T> c) {
+    const T h = T(0.70710678118654752440084436210484903928);
+    return INV ? cx<T>{-h * (c.x + c.y), h * (c.x - c.y)} : cx<T>{h * (c.y - c.x), -h * (c.x + c.y)};
+}
+
 template <bool INV, typename T> struct Bfly<8, INV, T> {
     static __device__ __forceinline__ void run(cx<T>* v) {
-        const T h = T(0.70710678118654752440084436210484903928);
-        // b_k = a_k + a_{k+4}; c_k = (a_k - a_{k+4}) W8^k
-        cx<T> b0 = cadd(v[0], v[4]), c0 = csub(v[0], v[4]);
-        cx<T> b1 = cadd(v[1], v[5]), c1 = csub(v[1], v[5]);
-        cx<T> b2 = cadd(v[2], v[6]), c2 = csub(v[2], v[6]);
-        cx<T> b3 = cadd(v[3], v[7]), c3 = csub(v[3], v[7]);
-        // W8^1 = h(1 - i) fwd / h(1 + i) inv ; W8^2 = -i / +i ; W8^3 = h(-1 - i) / h(-1 + i)
-        c1 = INV ? cx<T>{h * (c1.x - c1.y), h * (c1.x + c1.y)} : cx<T>{h * (c1.x + c1.y), h * (c1.y - c1.x)};
-        c2 = mul_mi<INV>(c2);
-        c3 = INV ? cx<T>{-h * (c3.x + c3.y), h * (c3.x - c3.y)} : cx<T>{h * (c3.y - c3.x), -h * (c3.x + c3.y)};
-        cx<T> e[4] = {b0, b1, b2, b3};
-        cx<T> o[4] = {c0, c1, c2, c3};
+        // b_k = a_k + a_{k+4}; c_k = (a_k - a_{k+4}) W8^k; Y[2m] = DFT4(b), Y[2m+1] = DFT4(c)
+        cx<T> e[4], o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            e[k] = cadd(v[k], v[k + 4]);
+            o[k] = csub(v[k], v[k + 4]);
+        }
+        o[1] = w8_1<INV, T>(o[1]);
+        o[2] = mul_mi<INV>(o[2]);
+        o[3] = w8_3<INV, T>(o[3]);
         Bfly<4, INV, T>::run(e);
         Bfly<4, INV, T>::run(o);
         v[0] = e[0]; v[2] = e[1]; v[4] = e[2]; v[6] = e[3];
         v[1] = o[0]; v[3] = o[1]; v[5] = o[2]; v[7] = o[3];
+    }
+};
+
+// 16-point DFT as 4 x 4: x[k1 + 4 k2]; DFT4 over k2, twiddle W16^{k1 m2}, DFT4 over k1.
+template <bool INV, typename T> struct Bfly<16, INV, T> {
+    static __device__ __forceinline__ void run(cx<T>* v) {
+        const T c1 = T(0.92387953251128675612818318939678828682);  // cos(pi/8)
+        const T s1 = T(0.38268343236508977172845998403039886676);  // sin(pi/8)
+        cx<T> a[4][4];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) a[k1][k2] = v[k1 + 4 * k2];
+            Bfly<4, INV, T>::run(a[k1]);  // a[k1][m2]
+        }
+        // twiddles W16^{k1 m2}; W16^1 = c1 - i s1 (fwd)
+        const T sg = INV ? T(1) : T(-1);
+        auto w1 = [&](cx<T> z) { return cx<T>{z.x * c1 - sg * z.y * s1, z.y * c1 + sg * z.x * s1}; };
+        auto w3 = [&](cx<T> z) { return cx<T>{z.x * s1 - sg * z.y * c1, z.y * s1 + sg * z.x * c1}; };
+        // W16^2 = W8^1, W16^4 = -i, W16^6 = W8^3, W16^9 = W16^8 * W16^1 = -W16^1
+        a[1][1] = w1(a[1][1]);
+        a[1][2] = w8_1<INV, T>(a[1][2]);
+        a[1][3] = w3(a[1][3]);
+        a[2][1] = w8_1<INV, T>(a[2][1]);
+        a[2][2] = mul_mi<INV>(a[2][2]);
+        a[2][3] = w8_3<INV, T>(a[2][3]);
+        a[3][1] = w3(a[3][1]);
+        a[3][2] = w8_3<INV, T>(a[3][2]);
+        {
+            cx<T> z = w1(a[3][3]);
+            a[3][3] = cx<T>{-z.x, -z.y};
+        }
+#pragma unroll
+        for (int m2 = 0; m2 < 4; ++m2) {
+            cx<T> b[4] = {a[0][m2], a[1][m2], a[2][m2], a[3][m2]};
+            Bfly<4, INV, T>::run(b);  // output m1: X[m2 + 4 m1]
+#pragma unroll
+            for (int m1 = 0; m1 < 4; ++m1) v[m2 + 4 * m1] = b[m1];
+        }
     }
 };
 
@@ -155,7 +204,6 @@ template <int R, bool INV, typename T> struct BflyOdd {
                 si += b[n].y * s;
             }
             // fwd: y_k = sum a cos - i sum b sin ; y_{R-k} = sum a cos + i sum b sin
-            // -i (sr + i si) = si - i sr
             if (!INV) {
                 out[k] = cx<T>{cr + si, ci - sr};
                 out[R - k] = cx<T>{cr - si, ci + sr};
@@ -174,38 +222,39 @@ template <bool INV, typename T> struct Bfly<11, INV, T> : BflyOdd<11, INV, T> {}
 template <bool INV, typename T> struct Bfly<13, INV, T> : BflyOdd<13, INV, T> {};
 
 // ------------------------------------------------------------------ plans
-// Radix of stage i for length Q: 8s first, then the 2-power remainder (4 or 2),
-// then 3, 5, 7, 11, 13.
+// Radix sequence for length q: 16s first, then the 2-power remainder (8, 4 or
+// 2), then 3, 5, 7, 11, 13.  REV reverses the order (used by the inverse of a
+// fused forward/inverse pair so the inverse's first stage reads exactly the
+// elements the forward's last stage left in each thread's registers).
+struct RadixPlan {
+    int n;
+    int r[12];
+};
 constexpr int pow2_part(int q) { int p = 1; while (q % 2 == 0) { q /= 2; p *= 2; } return p; }
-constexpr int num_stages(int q) {
-    int n = 0;
+constexpr RadixPlan make_plan(int q, bool rev, int maxr = 16) {
+    RadixPlan p{0, {0}};
     int p2 = pow2_part(q);
-    while (p2 >= 8) { p2 /= 8; ++n; }
-    if (p2 > 1) ++n;
-    q /= pow2_part(q);
-    const int odd[5] = {3, 5, 7, 11, 13};
-    for (int i = 0; i < 5; ++i) while (q % odd[i] == 0) { q /= odd[i]; ++n; }
-    return q == 1 ? n : -1;
-}
-constexpr int stage_radix(int q, int s) {
-    int p2 = pow2_part(q);
-    int i = 0;
-    while (p2 >= 8) { if (i == s) return 8; p2 /= 8; ++i; }
-    if (p2 > 1) { if (i == s) return p2; ++i; }
+    while (p2 >= maxr) { p.r[p.n++] = maxr; p2 /= maxr; }
+    if (p2 > 1) p.r[p.n++] = p2;
     q /= pow2_part(q);
     const int odd[5] = {3, 5, 7, 11, 13};
     for (int k = 0; k < 5; ++k)
-        while (q % odd[k] == 0) { if (i == s) return odd[k]; q /= odd[k]; ++i; }
-    return 0;
+        while (q % odd[k] == 0) { p.r[p.n++] = odd[k]; q /= odd[k]; }
+    if (q != 1) p.n = -1;  // unsupported prime factor
+    if (rev && p.n > 0)
+        for (int i = 0; i < p.n / 2; ++i) { int t = p.r[i]; p.r[i] = p.r[p.n - 1 - i]; p.r[p.n - 1 - i] = t; }
+    return p;
 }
-constexpr bool supported_length(int q) { return q >= 2 && num_stages(q) > 0; }
+constexpr bool supported_length(int q) { return q >= 2 && make_plan(q, false).n > 0; }
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+// threads per transform: one per maxr elements
+constexpr int tpf_for(int q, int maxr = 16) { return q / maxr > 0 ? q / maxr : 1; }
 // registers needed per thread: max over stages of (butterflies per thread) * radix
-constexpr int max_elems(int q, int tpf) {
+constexpr int max_elems(int q, int tpf, int maxr = 16) {
+    RadixPlan p = make_plan(q, false, maxr);
     int m = 0;
-    for (int s = 0; s < num_stages(q); ++s) {
-        int r = stage_radix(q, s);
-        int e = cdiv(q / r, tpf) * r;
+    for (int s = 0; s < p.n; ++s) {
+        int e = cdiv(q / p.r[s], tpf) * p.r[s];
         if (e > m) m = e;
     }
     return m;
@@ -213,38 +262,43 @@ constexpr int max_elems(int q, int tpf) {
 
 // ------------------------------------------------------------ the engine
 // LDS accessor concept: struct with  __device__ cx<T>& at(int elem)  for the
-// thread's own FFT.  TW: pointer to W_P^k table (k in [0,P)), TWS = P/Q.
+// thread's own transform.  tw: LDS (or global) table W_P^k, k in [0,P); the
+// length-Q stage twiddle W_Q^e is tw[e * tws] with tws = P / Q.
 //
 // Element ownership in a stage of radix R: butterfly j = t + b*TPF (b < NB),
 // inputs j + r*(Q/R), outputs ((j/NS)*NS*R + j%NS) + r*NS.
-template <typename T, int Q, int TPF> struct FftShape {
-    static constexpr int NST = num_stages(Q);
-    static constexpr int R0 = stage_radix(Q, 0);
-    static constexpr int RL = stage_radix(Q, NST - 1);
-    static constexpr int E = max_elems(Q, TPF);
+template <typename T, int Q, int TPF, bool REV, int MAXR = 16> struct FftShape {
+    static constexpr RadixPlan PL = make_plan(Q, REV, MAXR);
+    static constexpr int NST = PL.n;
+    static constexpr int R0 = PL.r[0];
+    static constexpr int RL = PL.r[PL.n - 1];
+    static constexpr int E = max_elems(Q, TPF, MAXR);
     static constexpr int NB0 = cdiv(Q / R0, TPF);
     static constexpr int NBL = cdiv(Q / RL, TPF);
     // first-stage input element of slot (b, r), or -1 if the slot is idle
+    // (t < TPF always; slots are all busy when TPF divides the butterfly count)
     static __device__ __forceinline__ int in_elem(int t, int b, int r) {
         int j = t + b * TPF;
+        if constexpr ((Q / R0) % TPF == 0) return j + r * (Q / R0);
         return (j < Q / R0) ? j + r * (Q / R0) : -1;
     }
     // last-stage output element of slot (b, r), or -1 if idle
     static __device__ __forceinline__ int out_elem(int t, int b, int r) {
         int j = t + b * TPF;
+        if constexpr ((Q / RL) % TPF == 0) return j + r * (Q / RL);
         return (j < Q / RL) ? j + r * (Q / RL) : -1;
     }
 };
 
-template <typename T, int Q, int TPF, bool INV, int S, int NS> struct StageRunner {
-    static constexpr int NST = num_stages(Q);
-    static constexpr int R = stage_radix(Q, S);
+template <typename T, int Q, int TPF, bool INV, bool REV, int S, int NS, int MAXR> struct StageRunner {
+    static constexpr RadixPlan PL = make_plan(Q, REV, MAXR);
+    static constexpr int NST = PL.n;
+    static constexpr int R = PL.r[S];
     static constexpr int NBF = Q / R;
     static constexpr int NB = cdiv(NBF, TPF);
 
     // butterflies of this stage on registers already holding the stage inputs
-    template <typename TW>
-    static __device__ __forceinline__ void compute(int t, cx<T>* v, const TW* tw, int tws) {
+    static __device__ __forceinline__ void compute(int t, cx<T>* v, const cx<T>* tw, int tws) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int j = t + b * TPF;
@@ -253,10 +307,7 @@ template <typename T, int Q, int TPF, bool INV, int S, int NS> struct StageRunne
                     const int k = j % NS;
                     const int step = (Q / (NS * R)) * tws;
 #pragma unroll
-                    for (int r = 1; r < R; ++r) {
-                        const TW w = tw[r * k * step];
-                        v[b * R + r] = ctw<INV>(v[b * R + r], cx<T>{T(w.x), T(w.y)});
-                    }
+                    for (int r = 1; r < R; ++r) v[b * R + r] = ctw<INV>(v[b * R + r], tw[r * k * step]);
                 }
                 Bfly<R, INV, T>::run(v + b * R);
             }
@@ -287,26 +338,94 @@ template <typename T, int Q, int TPF, bool INV, int S, int NS> struct StageRunne
     }
     // run this stage and the remaining ones.  On entry v holds this stage's
     // inputs (stage 0) or the data is in LDS (later stages, FROM_LDS).
-    template <bool FROM_LDS, typename LDS, typename TW>
-    static __device__ __forceinline__ void run(int t, cx<T>* v, LDS& lds, const TW* tw, int tws) {
+    template <bool FROM_LDS, typename LDS>
+    static __device__ __forceinline__ void run(int t, cx<T>* v, LDS& lds, const cx<T>* tw, int tws) {
         if (FROM_LDS) load(t, v, lds);
         compute(t, v, tw, tws);
         if constexpr (S + 1 < NST) {
             __syncthreads();  // every thread finished reading this stage's inputs
             store(t, v, lds);
             __syncthreads();  // stage outputs visible
-            StageRunner<T, Q, TPF, INV, S + 1, NS * R>::template run<true>(t, v, lds, tw, tws);
+            StageRunner<T, Q, TPF, INV, REV, S + 1, NS * R, MAXR>::template run<true>(t, v, lds, tw, tws);
         }
     }
 };
 
-// Full length-Q transform of one FFT per thread group.  On entry v holds the
-// first-stage inputs (FftShape::in_elem); on exit v holds the last-stage
-// outputs (FftShape::out_elem).  Contains __syncthreads(): every thread of the
-// work-group must call it.
-template <typename T, int Q, int TPF, bool INV, typename LDS, typename TW>
-__device__ __forceinline__ void fft_regs(int t, cx<T>* v, LDS& lds, const TW* tw, int tws) {
-    StageRunner<T, Q, TPF, INV, 0, 1>::template run<false>(t, v, lds, tw, tws);
+// Full length-Q transform of one transform per thread group.  On entry v holds
+// the first-stage inputs (FftShape<..,REV>::in_elem); on exit v holds the
+// last-stage outputs (FftShape<..,REV>::out_elem).  Contains __syncthreads():
+// every thread of the work-group must call it.
+template <typename T, int Q, int TPF, bool INV, bool REV, int MAXR = 16, typename LDS>
+__device__ __forceinline__ void fft_regs(int t, cx<T>* v, LDS& lds, const cx<T>* tw, int tws) {
+    StageRunner<T, Q, TPF, INV, REV, 0, 1, MAXR>::template run<false>(t, v, lds, tw, tws);
+}
+
+// ------------------------------------------------------- buffer memory ops
+// All HBM traffic of the passes goes through raw buffer instructions: a
+// per-array resource (base, size in bytes) and 32-bit byte offsets.  An offset
+// at or beyond the size reads 0 and drops stores, which is how masked lanes
+// are expressed (no branches).  Host code keeps every array below kBufLimit.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr unsigned kBufOOB = 0xFFFFFF00u;  // offset that is out of range of every resource
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct BufOps;
+template <> struct BufOps<float> {
+    static __device__ __forceinline__ float2 ld_cx(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st_cx(float2 v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    }
+    static __device__ __forceinline__ float ld_r(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st_r(float v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    }
+};
+template <> struct BufOps<double> {
+    static __device__ __forceinline__ double2 ld_cx(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st_cx(double2 v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+    }
+    static __device__ __forceinline__ double ld_r(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+    static __device__ __forceinline__ void st_r(double v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    }
+};
+
+// Copy a twiddle table (n entries) from global memory into LDS; caller syncs.
+template <typename V>
+__device__ __forceinline__ void stage_table(V* dst, const V* __restrict__ src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+// Same with a compile-time bound (NMAX entries at most, NT threads): every load
+// is issued before the first LDS store, so the table costs one memory latency
+// that overlaps whatever loads the caller issued before.
+template <int NMAX, int NT, typename V>
+__device__ __forceinline__ void stage_table_unrolled(V* dst, const V* __restrict__ src, int n) {
+    constexpr int IT = (NMAX + NT - 1) / NT;
+    V tmp[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int k = threadIdx.x + i * NT;
+        if (k < n) tmp[i] = src[k];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int k = threadIdx.x + i * NT;
+        if (k < n) dst[k] = tmp[i];
+    }
 }
 
 }  // namespace vfd

@@ -15,11 +15,12 @@ struct Plan {
     int L, hx;
     int rs_y, rs_z, qy, qz;
     bool full_box;
-    long ld, plane;
+    long ld, plane, plane_pad;
     long ks_row, ks_plane;
-    size_t work_bytes, kspec_bytes;
+    size_t work_bytes, kspec_bytes, field_bytes;
     double bytes_per_field_tick;
     const void *tw_x, *tw_y, *tw_z;
+    int var[5];  // schedule variant per pass
 };
 
 int set_error(int code, const char* fmt, ...);
@@ -27,6 +28,8 @@ int hip_check(hipError_t e, const char* what);
 int twiddles(int p, int prec, const void** out);
 
 int plan_init(const vfd_plan_desc* d, Plan* p);
+void tuned_lookup(Plan& q);
+int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms);
 int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
 int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st);
 double plan_pass_bytes(const Plan& p, int pass);

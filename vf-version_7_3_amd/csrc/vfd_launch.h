@@ -9,16 +9,19 @@ namespace vfd {
 bool length_supported(int q);
 
 // x-forward (pad + R2C) over Ny*Nz rows of nfields fields; L = Px/2.
+// `variant` selects a compiled schedule (0..3, see vfd_passes.hpp); lengths
+// without variants run schedule 0.
+constexpr int kScheduleVariants = 4;
 template <typename T>
 hipError_t launch_xfwd(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
-                       const void* tw, hipStream_t st);
+                       const void* tw, hipStream_t st, int variant = 0);
 // x-inverse (C2R + unpad).
 template <typename T>
 hipError_t launch_xinv(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
-                       const void* tw, hipStream_t st);
+                       const void* tw, hipStream_t st, int variant = 0);
 // column pass of sub-length Q (P = Q * cd.rs) over nplanes planes.
 template <typename T>
 hipError_t launch_col(int Q, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
-                      const void* tw, hipStream_t st);
+                      const void* tw, hipStream_t st, int variant = 0);
 
 }  // namespace vfd

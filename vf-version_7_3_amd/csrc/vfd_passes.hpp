@@ -20,25 +20,40 @@
 namespace vfd {
 
 // ------------------------------------------------------ launch geometry
-template <int Q> struct RowCfg {
-    static constexpr int E0 = 8;
-    static constexpr int TPF = (Q / E0) > 0 ? (Q / E0) : 1;
-    static constexpr int G = (256 / TPF) > 0 ? (256 / TPF) : 1;  // rows per block
+// Schedule variants (picked per pass by the autotuner):
+//   V = 0: radix <= 16, 16 elements per thread, blocks <= 512 threads
+//   V = 1: radix <= 8,   8 elements per thread, blocks <= 512 threads
+//   V = 2: radix <= 8,   8 elements per thread, blocks <= 256 threads
+//   V = 3: radix <= 16, 16 elements per thread, blocks <= 256 threads
+constexpr int kNumVariants = 4;
+constexpr int var_maxr(int v) { return (v == 1 || v == 2) ? 8 : 16; }
+constexpr int var_ntmax(int v) { return v >= 2 ? 256 : 512; }
+
+template <typename T, int Q, int V = 0> struct RowCfg {
+    static constexpr int MAXR = var_maxr(V);
+    static constexpr int TPF = tpf_for(Q, MAXR);
+    static constexpr int NTT = var_ntmax(V) / 2;                    // target threads
+    static constexpr int G = (NTT / TPF) > 0 ? (NTT / TPF) : 1;     // rows per group
     static constexpr int NT = TPF * G;
     static constexpr int PITCH = Q + Q / 8 + 1;                  // padded LDS row
+    // twiddle table W_Px (2Q entries) staged in LDS when it fits beside the rows
+    static constexpr bool TW_LDS = (2 * Q + G * PITCH) * (int)sizeof(cx<T>) <= 64 * 1024;
+    static constexpr int TW_N = TW_LDS ? 2 * Q : 1;
 };
 
-template <typename T, int Q> struct ColCfg {
-    static constexpr int E0 = 8;
-    static constexpr int TPF = (Q / E0) > 0 ? (Q / E0) : 1;
-    static constexpr int LDS_CAP = 80 * 1024;
+template <typename T, int Q, int V = 0> struct ColCfg {
+    static constexpr int MAXR = var_maxr(V);
+    static constexpr int TPF = tpf_for(Q, MAXR);
+    static constexpr int LDS_CAP = 72 * 1024;
     static constexpr int tc_for() {
         int tc = 64;
-        while (tc > 1 && (tc * TPF > 1024 || tc * Q * (int)sizeof(cx<T>) > LDS_CAP)) tc /= 2;
+        while (tc > 1 && (tc * TPF > var_ntmax(V) || (tc * Q + 2 * Q) * (int)sizeof(cx<T>) > LDS_CAP)) tc /= 2;
         return tc;
     }
     static constexpr int TC = tc_for();  // columns per block
     static constexpr int NT = TC * TPF;
+    // twiddle table W_P (P = Q * rs <= 2Q entries) staged in LDS
+    static constexpr int TW_N = 2 * Q;
 };
 
 // ------------------------------------------------------------ LDS views
@@ -51,68 +66,89 @@ template <typename T> struct RowLds {
     __device__ __forceinline__ cx<T>& at(int n) { return base[n + (n >> 3)]; }
 };
 
-template <typename V> __device__ __forceinline__ V tw_load(const V* tw, long k) { return tw[k]; }
-
 // ------------------------------------------------------------- x forward
 // rows r = z*Ny + y (z < Nz, y < Ny).  Packed z[n] = u(2n) + i u(2n+1) of the
 // padded row (length L = Px/2), length-L FFT, then the real-FFT post-process
 // A_k = 1/2 (Z_k + Z*_{L-k}) - i/2 W_Px^k (Z_k - Z*_{L-k}),  k = 0..L.
-template <typename T, int L>
-__global__ void __launch_bounds__(RowCfg<L>::NT)
-xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ tw) {
-    using C = RowCfg<L>;
-    using Sh = FftShape<T, L, C::TPF>;
+// Persistent: each work-group loops over groups of G rows of every field.
+template <typename T, int L, int V>
+__global__ void __launch_bounds__((RowCfg<T, L, V>::NT))
+xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int nfields) {
+    using C = RowCfg<T, L, V>;
+    using Sh = FftShape<T, L, C::TPF, false, C::MAXR>;
+    using B = BufOps<T>;
     __shared__ cx<T> smem[C::G * C::PITCH];
-    const FieldDesc& fd = fb.f[blockIdx.z];
+    __shared__ cx<T> twl[C::TW_N];
+    const cx<T>* tw = twg;
+    if constexpr (C::TW_LDS) {
+        stage_table(twl, twg, 2 * L);
+        __syncthreads();
+        tw = twl;
+    }
     const int t = threadIdx.x % C::TPF;
     const int gr = threadIdx.x / C::TPF;
-    const long row = (long)blockIdx.x * C::G + gr;
     const long nrows = (long)g.Ny * g.Nz;
-    const bool live = row < nrows;
-    const int z = live ? (int)(row / g.Ny) : 0;
-    const int y = live ? (int)(row % g.Ny) : 0;
-    const T* __restrict__ src = (const T*)fd.in + row * g.Nx;
-    // constant-pad mode: interior 0<z<D-1, 0<y<H-1, 0<x<epi keeps u-b, the rest 0
-    const bool cvf = g.pad_mode == PAD_CONSTANT_VF;
-    const bool row_in = live && (!cvf || (z > 0 && z < g.Nz - 1 && y > 0 && y < g.Ny - 1));
-    const int xhi = cvf ? (fd.epi < g.Nx ? fd.epi : g.Nx) : g.Nx;
-    const int xlo = cvf ? 1 : 0;
-    const T base = cvf ? (T)fd.base : T(0);
-
-    cx<T> v[Sh::E];
-#pragma unroll
-    for (int b = 0; b < Sh::NB0; ++b)
-#pragma unroll
-        for (int r = 0; r < Sh::R0; ++r) {
-            const int n = Sh::in_elem(t, b, r);
-            cx<T> val = {T(0), T(0)};
-            if (n >= 0 && row_in) {
-                const int x0 = 2 * n, x1 = 2 * n + 1;
-                if (x0 >= xlo && x0 < xhi) val.x = src[x0] - base;
-                if (x1 >= xlo && x1 < xhi) val.y = src[x1] - base;
-            }
-            v[b * Sh::R0 + r] = val;
-        }
+    const long ngroups = (nrows + C::G - 1) / C::G;
     RowLds<T> lds{smem + gr * C::PITCH};
-    fft_regs<T, L, C::TPF, false>(t, v, lds, tw, 2);
-    __syncthreads();
+    const bool cvf = g.pad_mode == PAD_CONSTANT_VF;
+    constexpr unsigned RB = sizeof(T), CB = sizeof(cx<T>);
+    for (long item = blockIdx.x; item < ngroups * nfields; item += gridDim.x) {
+        const int f = (int)(item / ngroups);
+        const long row = (item % ngroups) * C::G + gr;
+        const FieldDesc& fd = fb.f[f];
+        const rsrc_t rin = make_rsrc(fd.in, g.in_bytes);
+        const rsrc_t rwork = make_rsrc(fd.work, g.work_bytes);
+        const bool live = row < nrows;
+        const int z = live ? (int)(row / g.Ny) : 0;
+        const int y = live ? (int)(row % g.Ny) : 0;
+        const unsigned rowoff = (unsigned)(row * g.Nx) * RB;
+        // constant-pad mode: interior 0<z<D-1, 0<y<H-1, 0<x<epi keeps u-b, the rest 0
+        const bool row_in = live && (!cvf || (z > 0 && z < g.Nz - 1 && y > 0 && y < g.Ny - 1));
+        const int xhi = cvf ? (fd.epi < g.Nx ? fd.epi : g.Nx) : g.Nx;
+        const int xlo = cvf ? 1 : 0;
+        const T base = cvf ? (T)fd.base : T(0);
+        const bool vec = !cvf && (g.Nx % 2 == 0);
+
+        cx<T> v[Sh::E];
 #pragma unroll
-    for (int b = 0; b < Sh::NBL; ++b)
+        for (int b = 0; b < Sh::NB0; ++b)
 #pragma unroll
-        for (int r = 0; r < Sh::RL; ++r) {
-            const int k = Sh::out_elem(t, b, r);
-            if (k >= 0) lds.at(k) = v[b * Sh::RL + r];
+            for (int r = 0; r < Sh::R0; ++r) {
+                const int n = Sh::in_elem(t, b, r);
+                const int x0 = 2 * n, x1 = 2 * n + 1;
+                cx<T> val;
+                if (vec) {
+                    const bool ok = n >= 0 && row_in && x1 < g.Nx;
+                    val = B::ld_cx(rin, ok ? rowoff + (unsigned)x0 * RB : kBufOOB);
+                } else {
+                    const bool ok0 = n >= 0 && row_in && x0 >= xlo && x0 < xhi;
+                    const bool ok1 = n >= 0 && row_in && x1 >= xlo && x1 < xhi;
+                    const T a0 = B::ld_r(rin, ok0 ? rowoff + (unsigned)x0 * RB : kBufOOB);
+                    const T a1 = B::ld_r(rin, ok1 ? rowoff + (unsigned)x1 * RB : kBufOOB);
+                    val = cx<T>{ok0 ? a0 - base : T(0), ok1 ? a1 - base : T(0)};
+                }
+                v[b * Sh::R0 + r] = val;
+            }
+        fft_regs<T, L, C::TPF, false, false, C::MAXR>(t, v, lds, tw, 2);
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < Sh::NBL; ++b)
+#pragma unroll
+            for (int r = 0; r < Sh::RL; ++r) {
+                const int k = Sh::out_elem(t, b, r);
+                if (k >= 0) lds.at(k) = v[b * Sh::RL + r];
+            }
+        __syncthreads();
+        const unsigned dst = (unsigned)((long)z * g.plane + (long)y * g.ld) * CB;
+        for (int k = t; k <= L; k += C::TPF) {
+            const cx<T> zk = lds.at(k == L ? 0 : k);
+            const cx<T> zc = cconj(lds.at(k == 0 ? 0 : L - k));
+            const cx<T> e = cscale(cadd(zk, zc), T(0.5));
+            const cx<T> d = cscale(csub(zk, zc), T(0.5));
+            const cx<T> wd = cmul(d, tw[k]);  // W_Px^k d
+            B::st_cx(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (unsigned)k * CB : kBufOOB);  // e - i W d
         }
-    __syncthreads();
-    if (!live) return;
-    cx<T>* __restrict__ dst = (cx<T>*)fd.work + (long)z * g.plane + (long)y * g.ld;
-    for (int k = t; k <= L; k += C::TPF) {
-        const cx<T> zk = lds.at(k == L ? 0 : k);
-        const cx<T> zc = cconj(lds.at(k == 0 ? 0 : L - k));
-        const cx<T> e = cscale(cadd(zk, zc), T(0.5));
-        const cx<T> d = cscale(csub(zk, zc), T(0.5));
-        const cx<T> wd = cmul(d, tw_load(tw, k));     // W_Px^k d
-        dst[k] = cx<T>{e.x + wd.y, e.y - wd.x};       // e - i W d
+        // the next item's first LDS store happens after a barrier inside fft_regs
     }
 }
 
@@ -120,58 +156,77 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ tw) {
 // Z_k = (A_k + A*_{L-k}) + i (A_k - A*_{L-k}) W_Px^-k, length-L inverse FFT,
 // u(2n) = Re z_n, u(2n+1) = Im z_n (unnormalized C2R of length Px), crop to Nx,
 // plus base * K^(0,0,0) in constant-pad mode.
-template <typename T, int L>
-__global__ void __launch_bounds__(RowCfg<L>::NT)
-xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ tw) {
-    using C = RowCfg<L>;
-    using Sh = FftShape<T, L, C::TPF>;
+template <typename T, int L, int V>
+__global__ void __launch_bounds__((RowCfg<T, L, V>::NT))
+xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int nfields) {
+    using C = RowCfg<T, L, V>;
+    using Sh = FftShape<T, L, C::TPF, false, C::MAXR>;
+    using B = BufOps<T>;
     __shared__ cx<T> smem[C::G * C::PITCH];
-    const FieldDesc& fd = fb.f[blockIdx.z];
+    __shared__ cx<T> twl[C::TW_N];
+    const cx<T>* tw = twg;
+    if constexpr (C::TW_LDS) {
+        stage_table(twl, twg, 2 * L);
+        __syncthreads();
+        tw = twl;
+    }
     const int t = threadIdx.x % C::TPF;
     const int gr = threadIdx.x / C::TPF;
-    const long row = (long)blockIdx.x * C::G + gr;
     const long nrows = (long)g.Ny * g.Nz;
-    const bool live = row < nrows;
-    const int z = live ? (int)(row / g.Ny) : 0;
-    const int y = live ? (int)(row % g.Ny) : 0;
-    const cx<T>* __restrict__ srow = (const cx<T>*)fd.work + (long)z * g.plane + (long)y * g.ld;
-
-    cx<T> v[Sh::E];
-#pragma unroll
-    for (int b = 0; b < Sh::NB0; ++b)
-#pragma unroll
-        for (int r = 0; r < Sh::R0; ++r) {
-            const int k = Sh::in_elem(t, b, r);
-            cx<T> val = {T(0), T(0)};
-            if (k >= 0 && live) {
-                const cx<T> ak = srow[k];
-                const cx<T> ac = cconj(srow[L - k]);
-                const cx<T> xe = cadd(ak, ac);
-                const cx<T> xo = cmulc(csub(ak, ac), tw_load(tw, k));  // (A - A*) W^-k
-                val = cx<T>{xe.x - xo.y, xe.y + xo.x};                  // Xe + i Xo
-            }
-            v[b * Sh::R0 + r] = val;
-        }
+    const long ngroups = (nrows + C::G - 1) / C::G;
     RowLds<T> lds{smem + gr * C::PITCH};
-    fft_regs<T, L, C::TPF, true>(t, v, lds, tw, 2);
-    if (!live) return;
-    T addc = T(0);
-    if (g.pad_mode == PAD_CONSTANT_VF) {
-        const T dc = g.kspec_complex ? (T)((const cx<T>*)fd.kspec)[0].x : ((const T*)fd.kspec)[0];
-        addc = (T)fd.base * dc;
-    }
-    T* __restrict__ dst = (T*)fd.out + row * g.Nx;
+    constexpr unsigned RB = sizeof(T), CB = sizeof(cx<T>);
+    for (long item = blockIdx.x; item < ngroups * nfields; item += gridDim.x) {
+        const int f = (int)(item / ngroups);
+        const long row = (item % ngroups) * C::G + gr;
+        const FieldDesc& fd = fb.f[f];
+        const rsrc_t rwork = make_rsrc(fd.work, g.work_bytes);
+        const rsrc_t rout = make_rsrc(fd.out, g.out_bytes);
+        const bool live = row < nrows;
+        const int z = live ? (int)(row / g.Ny) : 0;
+        const int y = live ? (int)(row % g.Ny) : 0;
+        const unsigned srow = (unsigned)((long)z * g.plane + (long)y * g.ld) * CB;
+
+        cx<T> v[Sh::E];
 #pragma unroll
-    for (int b = 0; b < Sh::NBL; ++b)
+        for (int b = 0; b < Sh::NB0; ++b)
 #pragma unroll
-        for (int r = 0; r < Sh::RL; ++r) {
-            const int n = Sh::out_elem(t, b, r);
-            if (n >= 0) {
-                const cx<T> val = v[b * Sh::RL + r];
-                if (2 * n < g.Nx) dst[2 * n] = val.x + addc;
-                if (2 * n + 1 < g.Nx) dst[2 * n + 1] = val.y + addc;
+            for (int r = 0; r < Sh::R0; ++r) {
+                const int k = Sh::in_elem(t, b, r);
+                const bool ok = k >= 0 && live;
+                const cx<T> ak = B::ld_cx(rwork, ok ? srow + (unsigned)k * CB : kBufOOB);
+                const cx<T> ac = cconj(B::ld_cx(rwork, ok ? srow + (unsigned)(L - k) * CB : kBufOOB));
+                const cx<T> xe = cadd(ak, ac);
+                const cx<T> xo = cmulc(csub(ak, ac), tw[ok ? k : 0]);  // (A - A*) W^-k
+                v[b * Sh::R0 + r] = cx<T>{xe.x - xo.y, xe.y + xo.x};    // Xe + i Xo
             }
+        fft_regs<T, L, C::TPF, true, false, C::MAXR>(t, v, lds, tw, 2);
+        T addc = T(0);
+        if (g.pad_mode == PAD_CONSTANT_VF) {
+            const rsrc_t rk = make_rsrc(fd.kspec, g.kspec_bytes);
+            const T dc = B::ld_r(rk, 0);  // Re K^(0,0,0) (real part first in both layouts)
+            addc = (T)fd.base * dc;
         }
+        const unsigned orow = (unsigned)(row * g.Nx) * RB;
+        const bool vec = (g.Nx % 2 == 0);
+#pragma unroll
+        for (int b = 0; b < Sh::NBL; ++b)
+#pragma unroll
+            for (int r = 0; r < Sh::RL; ++r) {
+                const int n = Sh::out_elem(t, b, r);
+                const cx<T> val = v[b * Sh::RL + r];
+                if (vec) {
+                    const bool ok = n >= 0 && live && 2 * n + 1 < g.Nx;
+                    B::st_cx(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (unsigned)(2 * n) * RB : kBufOOB);
+                } else {
+                    const bool ok0 = n >= 0 && live && 2 * n < g.Nx;
+                    const bool ok1 = n >= 0 && live && 2 * n + 1 < g.Nx;
+                    B::st_r(val.x + addc, rout, ok0 ? orow + (unsigned)(2 * n) * RB : kBufOOB);
+                    B::st_r(val.y + addc, rout, ok1 ? orow + (unsigned)(2 * n + 1) * RB : kBufOOB);
+                }
+            }
+        // the next item's first LDS store happens after a barrier inside fft_regs
+    }
 }
 
 // ------------------------------------------------------------ column pass
@@ -182,139 +237,177 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ tw) {
 // (requires nin, nout <= Q), so the LDS strip holds Q rows instead of P.
 //   COL_FWD  : nin rows in, P rows out (natural frequency order)
 //   COL_INV  : P rows in, nout rows out
-//   COL_FUSED: nin rows in, forward, x K^[k]*scale, inverse, nout rows out
-template <typename T, int Q, int MODE, int RS>
-__global__ void __launch_bounds__((ColCfg<T, Q>::NT))
-col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ tw) {
-    using C = ColCfg<T, Q>;
-    using Sh = FftShape<T, Q, C::TPF>;
+//   COL_FUSED: nin rows in, forward, x K^[k]*scale, inverse, nout rows out.
+//              The inverse runs the reversed radix plan, so it starts on the
+//              elements the forward left in registers (no LDS redistribution).
+template <typename T, int Q, int MODE, int RS, int V>
+__global__ void __launch_bounds__((ColCfg<T, Q, V>::NT))
+col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
+    using C = ColCfg<T, Q, V>;
+    using ShF = FftShape<T, Q, C::TPF, false, C::MAXR>;                // forward / inverse-only plan
+    using ShI = FftShape<T, Q, C::TPF, MODE == COL_FUSED, C::MAXR>;    // inverse plan
+    using B = BufOps<T>;
+    constexpr int P = Q * RS;
+    constexpr unsigned CB = sizeof(cx<T>), RB = sizeof(T);
     __shared__ cx<T> smem[C::TC * Q];
+    __shared__ cx<T> tw[C::TW_N];
     const FieldDesc& fd = fb.f[blockIdx.z];
+    const rsrc_t rwork = make_rsrc(fd.work, cd.work_bytes);
     const int c = threadIdx.x % C::TC;
     const int t = threadIdx.x / C::TC;
     const long col = (long)blockIdx.x * C::TC + c;
     const bool live = col < cd.ncols;
-    cx<T>* __restrict__ base = (cx<T>*)fd.work + (long)blockIdx.y * cd.pstride + col;
+    const unsigned base = (unsigned)((long)blockIdx.y * cd.pstride + col) * CB;
+    const unsigned rstep = (unsigned)cd.rstride * CB;
     ColLds<T, C::TC> lds{smem + c};
-    constexpr int NI = Sh::NB0 * Sh::R0;
-    constexpr int NO = Sh::NBL * Sh::RL;
+    constexpr int NI = ShF::NB0 * ShF::R0;
+    constexpr int NO = ShI::NBL * ShI::RL;
+    constexpr int NK = ShF::NBL * ShF::RL;
 
-    cx<T> x[MODE == COL_INV ? 1 : NI];   // saved inputs (forward modes)
+    constexpr int NII = ShI::NB0 * ShI::R0;
+    cx<T> x[MODE == COL_INV ? NII * RS : NI];   // saved inputs (all sub-sequences in INV mode)
     cx<T> acc[MODE == COL_FWD ? 1 : NO]; // accumulated outputs (inverse modes)
-    cx<T> v[Sh::E];
+    cx<T> v[ShF::E > ShI::E ? ShF::E : ShI::E];
 
     if constexpr (MODE != COL_INV) {
 #pragma unroll
-        for (int b = 0; b < Sh::NB0; ++b)
+        for (int b = 0; b < ShF::NB0; ++b)
 #pragma unroll
-            for (int r = 0; r < Sh::R0; ++r) {
-                const int n = Sh::in_elem(t, b, r);
-                cx<T> val = {T(0), T(0)};
-                if (n >= 0 && n < cd.nin && live) val = base[(long)n * cd.rstride];
-                x[b * Sh::R0 + r] = val;
+            for (int r = 0; r < ShF::R0; ++r) {
+                const int n = ShF::in_elem(t, b, r);
+                const bool ok = n >= 0 && n < cd.nin && live;
+                x[b * ShF::R0 + r] = B::ld_cx(rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
             }
+    }
+    if constexpr (MODE == COL_INV) {
+        // inverse-only: every sub-sequence X[m*RS + s] is loaded up front
+#pragma unroll
+        for (int s = 0; s < RS; ++s)
+#pragma unroll
+            for (int b = 0; b < ShI::NB0; ++b)
+#pragma unroll
+                for (int r = 0; r < ShI::R0; ++r) {
+                    const int m = ShI::in_elem(t, b, r);
+                    const bool ok = m >= 0 && live;
+                    x[s * NII + b * ShI::R0 + r] = B::ld_cx(rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
+                }
     }
     if constexpr (MODE != COL_FWD) {
 #pragma unroll
         for (int i = 0; i < NO; ++i) acc[i] = cx<T>{T(0), T(0)};
     }
 
+    // kernel spectrum for the fused mode, prefetched before any transform work:
+    // real K^ stores the RS frequencies m*RS + s of one element side by side
+    // (one load per element), complex K^ is the R2C layout (one load per s)
+    [[maybe_unused]] cx<T> kv[MODE == COL_FUSED ? NK : 1];
+    [[maybe_unused]] cx<T> kvc[MODE == COL_FUSED && RS > 1 ? NK : 1];
+    if constexpr (MODE == COL_FUSED) {
+        const rsrc_t rk = make_rsrc(fd.kspec, cd.kspec_bytes);
+        if (cd.kspec_complex) {
+            const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col) * CB;
+            const unsigned kstep = (unsigned)cd.ks_row * CB;
 #pragma unroll
-    for (int s = 0; s < RS; ++s) {
-        if constexpr (MODE != COL_INV) {
-            // forward sub-transform s on the saved inputs
+            for (int b = 0; b < ShF::NBL; ++b)
 #pragma unroll
-            for (int b = 0; b < Sh::NB0; ++b)
-#pragma unroll
-                for (int r = 0; r < Sh::R0; ++r) {
-                    const int n = Sh::in_elem(t, b, r);
-                    cx<T> val = x[b * Sh::R0 + r];
-                    if (s && n >= 0) val = cmul(val, tw_load(tw, n));  // W_P^n
-                    v[b * Sh::R0 + r] = val;
-                }
-            fft_regs<T, Q, C::TPF, false>(t, v, lds, tw, RS);
-        }
-        if constexpr (MODE == COL_FWD) {
-#pragma unroll
-            for (int b = 0; b < Sh::NBL; ++b)
-#pragma unroll
-                for (int r = 0; r < Sh::RL; ++r) {
-                    const int m = Sh::out_elem(t, b, r);
-                    if (m >= 0 && live) base[(long)(m * RS + s) * cd.rstride] = v[b * Sh::RL + r];
+                for (int r = 0; r < ShF::RL; ++r) {
+                    const int m = ShF::out_elem(t, b, r);
+                    const bool ok = m >= 0 && live;
+                    kv[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (unsigned)(m * RS) * kstep : kBufOOB);
+                    if constexpr (RS > 1)
+                        kvc[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (unsigned)(m * RS + 1) * kstep : kBufOOB);
                 }
         } else {
-            if constexpr (MODE == COL_FUSED) {
-                // spectral multiply by K^ (frequency k = m*RS + s along this axis)
-                const long kbase = (long)blockIdx.y * cd.ks_plane + col;
+            const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col * RS) * RB;
+            const unsigned kstep = (unsigned)cd.ks_row * RB;
 #pragma unroll
-                for (int b = 0; b < Sh::NBL; ++b)
+            for (int b = 0; b < ShF::NBL; ++b)
 #pragma unroll
-                    for (int r = 0; r < Sh::RL; ++r) {
-                        const int m = Sh::out_elem(t, b, r);
-                        if (m >= 0) {
-                            const long ki = kbase + (long)(m * RS + s) * cd.ks_row;
-                            cx<T> val = v[b * Sh::RL + r];
-                            if (live) {
-                                if (cd.kspec_complex) {
-                                    val = cmul(val, ((const cx<T>*)fd.kspec)[ki]);
-                                } else {
-                                    val = cscale(val, ((const T*)fd.kspec)[ki]);
-                                }
-                            }
-                            v[b * Sh::RL + r] = cscale(val, (T)cd.scale);
-                        }
-                    }
-                // re-distribute: last-stage ownership -> first-stage ownership
-                __syncthreads();
+                for (int r = 0; r < ShF::RL; ++r) {
+                    const int m = ShF::out_elem(t, b, r);
+                    const bool ok = m >= 0 && live;
+                    const unsigned off = ok ? kbase + (unsigned)m * kstep : kBufOOB;
+                    if constexpr (RS > 1) kv[b * ShF::RL + r] = B::ld_cx(rk, off);  // (K^[2m], K^[2m+1])
+                    else kv[b * ShF::RL + r] = cx<T>{B::ld_r(rk, off), T(0)};
+                }
+        }
+    }
+
+    // twiddle table: loads issued after the data loads, one latency for both
+    stage_table_unrolled<C::TW_N, C::NT>(tw, twg, P);
+    __syncthreads();
+
 #pragma unroll
-                for (int b = 0; b < Sh::NBL; ++b)
+    for (int s = 0; s < RS; ++s) {
+        if constexpr (MODE == COL_FUSED) {
 #pragma unroll
-                    for (int r = 0; r < Sh::RL; ++r) {
-                        const int m = Sh::out_elem(t, b, r);
-                        if (m >= 0) lds.at(m) = v[b * Sh::RL + r];
-                    }
-                __syncthreads();
+            for (int b = 0; b < ShF::NB0; ++b)
 #pragma unroll
-                for (int b = 0; b < Sh::NB0; ++b)
+                for (int r = 0; r < ShF::R0; ++r) {
+                    const int n = ShF::in_elem(t, b, r);
+                    cx<T> val = x[b * ShF::R0 + r];
+                    if (s) val = cmul(val, tw[n >= 0 ? n : 0]);  // W_P^n
+                    v[b * ShF::R0 + r] = val;
+                }
+            fft_regs<T, Q, C::TPF, false, false, C::MAXR>(t, v, lds, tw, RS);
+            if (cd.kspec_complex) {
+                const T sc = (T)cd.scale;
 #pragma unroll
-                    for (int r = 0; r < Sh::R0; ++r) {
-                        const int m = Sh::in_elem(t, b, r);
-                        if (m >= 0) v[b * Sh::R0 + r] = lds.at(m);
-                    }
+                for (int i = 0; i < NK; ++i) {
+                    cx<T> k = kv[i];
+                    if constexpr (RS > 1) k = s ? kvc[i] : kv[i];
+                    v[i] = cscale(cmul(v[i], k), sc);
+                }
             } else {
-                // inverse-only: load sub-sequence X[m*RS + s]
 #pragma unroll
-                for (int b = 0; b < Sh::NB0; ++b)
-#pragma unroll
-                    for (int r = 0; r < Sh::R0; ++r) {
-                        const int m = Sh::in_elem(t, b, r);
-                        cx<T> val = {T(0), T(0)};
-                        if (m >= 0 && live) val = base[(long)(m * RS + s) * cd.rstride];
-                        v[b * Sh::R0 + r] = val;
-                    }
+                for (int i = 0; i < NK; ++i) v[i] = cscale(v[i], s ? kv[i].y : kv[i].x);
             }
-            fft_regs<T, Q, C::TPF, true>(t, v, lds, tw, RS);
+            fft_regs<T, Q, C::TPF, true, true, C::MAXR>(t, v, lds, tw, RS);
+        } else if constexpr (MODE == COL_FWD) {
 #pragma unroll
-            for (int b = 0; b < Sh::NBL; ++b)
+            for (int b = 0; b < ShF::NB0; ++b)
 #pragma unroll
-                for (int r = 0; r < Sh::RL; ++r) {
-                    const int n = Sh::out_elem(t, b, r);
-                    if (n >= 0) {
-                        cx<T> val = v[b * Sh::RL + r];
-                        if (s) val = cmulc(val, tw_load(tw, n));  // W_P^-n
-                        acc[b * Sh::RL + r] = cadd(acc[b * Sh::RL + r], val);
-                    }
+                for (int r = 0; r < ShF::R0; ++r) {
+                    const int n = ShF::in_elem(t, b, r);
+                    cx<T> val = x[b * ShF::R0 + r];
+                    if (s) val = cmul(val, tw[n >= 0 ? n : 0]);  // W_P^n
+                    v[b * ShF::R0 + r] = val;
+                }
+            fft_regs<T, Q, C::TPF, false, false, C::MAXR>(t, v, lds, tw, RS);
+#pragma unroll
+            for (int b = 0; b < ShF::NBL; ++b)
+#pragma unroll
+                for (int r = 0; r < ShF::RL; ++r) {
+                    const int m = ShF::out_elem(t, b, r);
+                    const bool ok = m >= 0 && live;
+                    B::st_cx(v[b * ShF::RL + r], rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
+                }
+            continue;
+        } else {
+#pragma unroll
+            for (int i = 0; i < NII; ++i) v[i] = x[s * NII + i];
+            fft_regs<T, Q, C::TPF, true, false, C::MAXR>(t, v, lds, tw, RS);
+        }
+        if constexpr (MODE != COL_FWD) {
+#pragma unroll
+            for (int b = 0; b < ShI::NBL; ++b)
+#pragma unroll
+                for (int r = 0; r < ShI::RL; ++r) {
+                    const int n = ShI::out_elem(t, b, r);
+                    cx<T> val = v[b * ShI::RL + r];
+                    if (s) val = cmulc(val, tw[n >= 0 ? n : 0]);  // W_P^-n
+                    acc[b * ShI::RL + r] = cadd(acc[b * ShI::RL + r], val);
                 }
         }
     }
     if constexpr (MODE != COL_FWD) {
-        if (!live) return;
 #pragma unroll
-        for (int b = 0; b < Sh::NBL; ++b)
+        for (int b = 0; b < ShI::NBL; ++b)
 #pragma unroll
-            for (int r = 0; r < Sh::RL; ++r) {
-                const int n = Sh::out_elem(t, b, r);
-                if (n >= 0 && n < cd.nout) base[(long)n * cd.rstride] = acc[b * Sh::RL + r];
+            for (int r = 0; r < ShI::RL; ++r) {
+                const int n = ShI::out_elem(t, b, r);
+                const bool ok = n >= 0 && n < cd.nout && live;
+                B::st_cx(acc[b * ShI::RL + r], rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
             }
     }
 }

@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -84,8 +86,41 @@ int twiddles(int p, int prec, const void** out) {
     return VFD_OK;
 }
 
+// ------------------------------------------------------- schedule cache
+// Tuned per-pass schedule variants, keyed by device and plan shape.
+struct ShapeKey {
+    int dev, prec, pad, kfmt, nz, ny, nx, pz, py, px;
+    bool operator<(const ShapeKey& o) const {
+        const int a[10] = {dev, prec, pad, kfmt, nz, ny, nx, pz, py, px};
+        const int b[10] = {o.dev, o.prec, o.pad, o.kfmt, o.nz, o.ny, o.nx, o.pz, o.py, o.px};
+        for (int i = 0; i < 10; ++i)
+            if (a[i] != b[i]) return a[i] < b[i];
+        return false;
+    }
+};
+static std::mutex g_sched_mu;
+static std::map<ShapeKey, std::array<int, 5>> g_sched;
+
+static ShapeKey shape_key(const Plan& q) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return ShapeKey{dev, q.prec, q.pad_mode, q.kspec_format, q.nz, q.ny, q.nx, q.pz, q.py, q.px};
+}
+
+void tuned_lookup(Plan& q) {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    auto it = g_sched.find(shape_key(q));
+    if (it != g_sched.end())
+        for (int i = 0; i < 5; ++i) q.var[i] = it->second[i];
+}
+
 // ------------------------------------------------------------------ plans
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+static long env_long(const char* name, long dflt) {
+    const char* s = getenv(name);
+    return (s && *s) ? strtol(s, nullptr, 10) : dflt;
+}
 
 int plan_init(const vfd_plan_desc* d, Plan* p) {
     if (!d) return set_error(VFD_ERR_INVALID, "plan: null descriptor");
@@ -129,19 +164,31 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     if (!pick(q.nz, q.pz, &q.rs_z, &q.qz))
         return set_error(VFD_ERR_UNSUPPORTED, "plan: z transform length %d not compiled in", q.pz);
     q.full_box = (q.nx == q.px && q.ny == q.py && q.nz == q.pz);
-    q.ld = q.full_box ? q.hx : round_up(q.hx, 16);
-    q.plane = (long)q.py * q.ld;
+    // rows start on 128-byte lines for both the complex work rows and the real
+    // kernel-spectrum rows (32 elements)
+    q.ld = q.full_box ? q.hx : round_up(q.hx, 32);
+    // Plane strides that are multiples of a large power of two put every row
+    // of a z-column on the same HBM channel; a small pad spreads them.
+    q.plane_pad = q.full_box ? 0 : env_long("VFD_PLANE_PAD", 0);
+    q.plane = (long)q.py * q.ld + q.plane_pad;
     const size_t cxb = 2 * (size_t)q.prec;
     q.work_bytes = (size_t)q.nz * q.plane * cxb;
     if (q.kspec_format == VFD_KSPEC_REAL) {
-        q.ks_row = (long)q.py * q.ld;  // along kz
-        q.ks_plane = q.ld;             // along ky
+        // element (kz,ky,kx) at ((kz/rs)*py + ky)*ld*rs + kx*rs + kz%rs: the rs
+        // frequencies one z sub-transform pair needs sit side by side
+        q.ks_row = (long)q.py * q.ld * q.rs_z;  // per kz/rs
+        q.ks_plane = q.ld * q.rs_z;             // per ky
         q.kspec_bytes = (size_t)q.pz * q.py * q.ld * q.prec;
     } else {
         q.ks_row = (long)q.py * q.hx;
         q.ks_plane = q.hx;
         q.kspec_bytes = (size_t)q.pz * q.py * q.hx * cxb;
     }
+    q.field_bytes = (size_t)q.nx * q.ny * q.nz * q.prec;
+    if (q.work_bytes >= kBufLimit || q.kspec_bytes >= kBufLimit || q.field_bytes >= kBufLimit)
+        return set_error(VFD_ERR_UNSUPPORTED,
+                         "plan: a per-field array exceeds 4 GiB (work %zu, spectrum %zu, field %zu bytes)",
+                         q.work_bytes, q.kspec_bytes, q.field_bytes);
     // pruned 5-pass algorithmic bytes: 8N + 32 A1 + 32 A2 + 4 A3 (FP32); the
     // complex terms scale with the precision and K^ with its element size.
     const double N = (double)q.nx * q.ny * q.nz;
@@ -150,6 +197,16 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     const double r = (double)q.prec, c = 2.0 * q.prec;
     const double kel = q.kspec_format == VFD_KSPEC_REAL ? r : c;
     q.bytes_per_field_tick = 2 * r * N + 4 * c * A1 + 4 * c * A2 + kel * A3;
+    // schedule variants per pass: tuned choice for this shape if known, else
+    // VFD_SCHEDULE="x,y,z,yi,xi" or schedule 0
+    for (int i = 0; i < 5; ++i) q.var[i] = 0;
+    if (const char* sv = getenv("VFD_SCHEDULE")) {
+        int v[5];
+        if (sscanf(sv, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5)
+            for (int i = 0; i < 5; ++i) q.var[i] = (v[i] >= 0 && v[i] < kScheduleVariants) ? v[i] : 0;
+    } else {
+        tuned_lookup(q);
+    }
     if (int rc = twiddles(q.px, q.prec, &q.tw_x)) return rc;
     if (int rc = twiddles(q.py, q.prec, &q.tw_y)) return rc;
     if (int rc = twiddles(q.pz, q.prec, &q.tw_z)) return rc;
@@ -174,25 +231,29 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             fb.f[i].base = f.base;
             fb.f[i].epi = f.epi;
         }
-        GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD};
+        GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD,
+                   p.field_bytes, p.work_bytes, p.kspec_bytes, p.field_bytes};
         ColDesc y{};
         y.mode = COL_FWD; y.rs = p.rs_y; y.nin = p.ny; y.nout = p.py; y.ncols = p.hx;
         y.rstride = p.ld; y.pstride = p.plane; y.scale = 1.0;
+        y.work_bytes = p.work_bytes; y.kspec_bytes = p.kspec_bytes;
         ColDesc z{};
         z.mode = COL_FUSED; z.rs = p.rs_z; z.nin = p.nz; z.nout = p.nz; z.ncols = p.hx;
         z.rstride = p.plane; z.pstride = p.ld; z.ks_row = p.ks_row; z.ks_plane = p.ks_plane;
         z.kspec_complex = p.kspec_format == VFD_KSPEC_COMPLEX_STD;
-        z.scale = 1.0 / ((double)p.px * p.py * p.pz);
+        // the real native spectrum carries the 1/(px py pz) normalization
+        z.scale = z.kspec_complex ? 1.0 / ((double)p.px * p.py * p.pz) : 1.0;
+        z.work_bytes = p.work_bytes; z.kspec_bytes = p.kspec_bytes;
         ColDesc yi = y;
         yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
         for (int pass = pass_lo; pass < pass_hi; ++pass) {
             hipError_t e = hipSuccess;
             switch (pass) {
-                case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st); break;
-                case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st); break;
-                case VFD_PASS_ZCONV: e = launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st); break;
-                case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st); break;
-                case VFD_PASS_XINV: e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st); break;
+                case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
+                case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st, p.var[pass]); break;
+                case VFD_PASS_ZCONV: e = launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[pass]); break;
+                case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
+                case VFD_PASS_XINV: e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
             }
             if (e != hipSuccess) return hip_check(e, "pass launch");
         }
@@ -236,6 +297,58 @@ int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_
     return plan_diffuse_passes(p, fields, nfields, 0, 5, st);
 }
 
+// Autotuner: time every compiled schedule variant of every pass on scratch
+// buffers of nfields fields (zero-filled; timing does not depend on values),
+// keep the fastest per pass, and remember it for this shape.
+int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms) {
+    if (nfields < 1) nfields = 1;
+    if (nfields > kMaxFieldsPerLaunch) nfields = kMaxFieldsPerLaunch;
+    const size_t fbytes = (size_t)p.nx * p.ny * p.nz * p.prec;
+    const size_t total = fbytes + p.kspec_bytes + (size_t)nfields * p.work_bytes;
+    char* buf = nullptr;
+    if (int rc = hip_check(hipMalloc((void**)&buf, total), "autotune scratch")) return rc;
+    int rc = hip_check(hipMemsetAsync(buf, 0, total, st), "autotune clear");
+    std::vector<vfd_field> fl(nfields);
+    for (int i = 0; i < nfields; ++i)
+        fl[i] = vfd_field{buf, buf, buf + fbytes + p.kspec_bytes + (size_t)i * p.work_bytes, buf + fbytes, 0.0, 0, 0};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = hip_check(hipEventCreate(&e0), "event");
+    if (!rc) rc = hip_check(hipEventCreate(&e1), "event");
+    const int saved[5] = {p.var[0], p.var[1], p.var[2], p.var[3], p.var[4]};
+    for (int pass = 0; pass < 5 && !rc; ++pass) {
+        float best = 1e30f;
+        int bestv = saved[pass];
+        for (int v = 0; v < kScheduleVariants && !rc; ++v) {
+            p.var[pass] = v;
+            rc = plan_diffuse_passes(p, fl.data(), nfields, pass, pass + 1, st);  // warm-up
+            if (rc) break;
+            rc = hip_check(hipEventRecord(e0, st), "event");
+            for (int k = 0; k < 3 && !rc; ++k) rc = plan_diffuse_passes(p, fl.data(), nfields, pass, pass + 1, st);
+            if (!rc) rc = hip_check(hipEventRecord(e1, st), "event");
+            if (!rc) rc = hip_check(hipEventSynchronize(e1), "event sync");
+            float ms = 0.f;
+            if (!rc) rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            if (!rc && ms < best) {
+                best = ms;
+                bestv = v;
+            }
+        }
+        p.var[pass] = bestv;
+        if (best_ms) best_ms[pass] = best / 3.0f;
+    }
+    if (rc)
+        for (int i = 0; i < 5; ++i) p.var[i] = saved[i];
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    if (!rc) {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        g_sched[shape_key(p)] = {p.var[0], p.var[1], p.var[2], p.var[3], p.var[4]};
+    }
+    return rc;
+}
+
 // Standalone full-box transforms (used by the kernel-spectrum precompute).
 template <typename T>
 static int run_r2c(const Plan& p, const void* in, void* out, hipStream_t st) {
@@ -243,12 +356,14 @@ static int run_r2c(const Plan& p, const void* in, void* out, hipStream_t st) {
     memset(&fb, 0, sizeof fb);
     fb.f[0].in = in;
     fb.f[0].work = out;
-    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0};
+    const size_t cbytes = (size_t)p.pz * p.py * p.hx * 2 * p.prec;
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, p.field_bytes, cbytes, 0, 0};
     hipError_t e = launch_xfwd<T>(p.L, fb, 1, g, p.tw_x, st);
     if (e != hipSuccess) return hip_check(e, "r2c x launch");
     ColDesc y{};
     y.mode = COL_FWD; y.rs = 1; y.nin = p.ny; y.nout = p.py; y.ncols = p.hx;
     y.rstride = p.ld; y.pstride = p.plane; y.scale = 1.0;
+    y.work_bytes = cbytes;
     e = launch_col<T>(p.py, fb, 1, y, p.nz, p.tw_y, st);
     if (e != hipSuccess) return hip_check(e, "r2c y launch");
     ColDesc z = y;
@@ -263,16 +378,18 @@ static int run_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
     memset(&fb, 0, sizeof fb);
     fb.f[0].out = out;
     fb.f[0].work = in;
+    const size_t cbytes = (size_t)p.pz * p.py * p.hx * 2 * p.prec;
     ColDesc z{};
     z.mode = COL_INV; z.rs = 1; z.nin = p.pz; z.nout = p.pz; z.ncols = p.hx;
     z.rstride = p.plane; z.pstride = p.ld; z.scale = 1.0;
+    z.work_bytes = cbytes;
     hipError_t e = launch_col<T>(p.pz, fb, 1, z, p.py, p.tw_z, st);
     if (e != hipSuccess) return hip_check(e, "c2r z launch");
     ColDesc y = z;
     y.nin = p.py; y.nout = p.py; y.rstride = p.ld; y.pstride = p.plane;
     e = launch_col<T>(p.py, fb, 1, y, p.nz, p.tw_y, st);
     if (e != hipSuccess) return hip_check(e, "c2r y launch");
-    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0};
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, 0, cbytes, 0, p.field_bytes};
     e = launch_xinv<T>(p.L, fb, 1, g, p.tw_x, st);
     return e == hipSuccess ? VFD_OK : hip_check(e, "c2r x launch");
 }
@@ -283,14 +400,18 @@ static int run_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
 template <typename T>
 __global__ void gauss_product_kernel(T* __restrict__ out, const double* __restrict__ gx,
                                      const double* __restrict__ gy, const double* __restrict__ gz,
-                                     int hx, long ld, int py, int pz, double decay) {
+                                     int hx, long ld, int py, int pz, int rs, double scale) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long total = (long)pz * py * ld;
     if (i >= total) return;
-    const long kx = i % ld;
-    const long r = i / ld;
-    const int ky = (int)(r % py), kz = (int)(r / py);
-    out[i] = kx < hx ? (T)(decay * gz[kz] * gy[ky] * gx[kx]) : T(0);
+    // storage order ((kz/rs)*py + ky)*ld*rs + kx*rs + kz%rs
+    const int s = (int)(i % rs);
+    const long rest = i / rs;
+    const long kx = rest % ld;
+    const long r2 = rest / ld;
+    const int ky = (int)(r2 % py);
+    const int kz = (int)(r2 / py) * rs + s;
+    out[i] = kx < hx ? (T)(scale * gz[kz] * gy[ky] * gx[kx]) : T(0);
 }
 
 static void gauss_dft(double sigma2, int p, int nk, std::vector<double>& out) {
@@ -333,12 +454,13 @@ int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspe
     if (!rc) {
         const long total = (long)p.pz * p.py * p.ld;
         const unsigned blocks = (unsigned)((total + 255) / 256);
+        const double scale = decay / ((double)p.px * p.py * p.pz);
         if (p.prec == VFD_F32)
             hipLaunchKernelGGL(gauss_product_kernel<float>, dim3(blocks), dim3(256), 0, st, (float*)d_kspec, d,
-                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, decay);
+                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, p.rs_z, scale);
         else
             hipLaunchKernelGGL(gauss_product_kernel<double>, dim3(blocks), dim3(256), 0, st, (double*)d_kspec, d,
-                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, decay);
+                               d + gx.size(), d + gx.size() + gy.size(), p.hx, p.ld, p.py, p.pz, p.rs_z, scale);
         rc = hip_check(hipGetLastError(), "gauss spectrum launch");
         // the host vector must outlive the async copy
         if (!rc) rc = hip_check(hipStreamSynchronize(st), "gauss spectrum sync");
@@ -503,6 +625,26 @@ int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields,
 }
 
 double vfd_plan_pass_bytes(const vfd_plan* plan, int pass) { return plan ? plan_pass_bytes(plan->p, pass) : 0.0; }
+
+int vfd_plan_autotune(vfd_plan* plan, int nfields, void* stream, float* best_ms) {
+    if (!plan) return set_error(VFD_ERR_INVALID, "plan_autotune: null plan");
+    return plan_autotune(plan->p, nfields, (hipStream_t)stream, best_ms);
+}
+
+int vfd_plan_get_schedule(const vfd_plan* plan, int* variants) {
+    if (!plan || !variants) return set_error(VFD_ERR_INVALID, "plan_get_schedule: null argument");
+    for (int i = 0; i < 5; ++i) variants[i] = plan->p.var[i];
+    return VFD_OK;
+}
+
+int vfd_plan_set_schedule(vfd_plan* plan, const int* variants) {
+    if (!plan || !variants) return set_error(VFD_ERR_INVALID, "plan_set_schedule: null argument");
+    for (int i = 0; i < 5; ++i)
+        if (variants[i] < 0 || variants[i] >= kScheduleVariants)
+            return set_error(VFD_ERR_INVALID, "plan_set_schedule: variant %d out of range", variants[i]);
+    for (int i = 0; i < 5; ++i) plan->p.var[i] = variants[i];
+    return VFD_OK;
+}
 
 int vfd_gauss_spectrum(const vfd_plan* plan, double sigma2, double decay, void* d_kspec, void* stream) {
     if (!plan) return set_error(VFD_ERR_INVALID, "gauss_spectrum: null plan");

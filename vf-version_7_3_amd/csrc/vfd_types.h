@@ -35,6 +35,7 @@ struct GridDesc {
     long plane;         // work plane stride = Py * ld
     int pad_mode;       // PadMode
     int kspec_complex;  // 1: complex spectrum, 0: real
+    unsigned long in_bytes, work_bytes, kspec_bytes, out_bytes;  // per-field array sizes
 };
 
 // One column pass (y or z).
@@ -49,6 +50,10 @@ struct ColDesc {
     long ks_plane;      // kernel-spectrum stride along the plane index
     int kspec_complex;
     double scale;       // applied to K^ products (1 / (Px Py Pz))
+    unsigned long work_bytes, kspec_bytes;  // per-field array sizes
 };
+
+// Every per-field array is addressed with 32-bit byte offsets.
+constexpr unsigned long kBufLimit = 0xFFFFFF00ul;
 
 }  // namespace vfd

@@ -63,6 +63,9 @@ lib.vfd_diffuse.argtypes = [_p, ctypes.POINTER(Field), _i, _p]
 lib.vfd_diffuse_pass.argtypes = [_p, ctypes.POINTER(Field), _i, _i, _p]
 lib.vfd_plan_pass_bytes.argtypes = [_p, _i]
 lib.vfd_plan_pass_bytes.restype = _d
+lib.vfd_plan_autotune.argtypes = [_p, _i, _p, ctypes.POINTER(ctypes.c_float)]
+lib.vfd_plan_get_schedule.argtypes = [_p, ctypes.POINTER(_i)]
+lib.vfd_plan_set_schedule.argtypes = [_p, ctypes.POINTER(_i)]
 lib.vfd_gauss_spectrum.argtypes = [_p, _d, _d, _p, _p]
 lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _p]
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
@@ -109,7 +112,7 @@ class Plan:
         self.kspec_format = kspec_format
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:
             lib.vfd_plan_destroy(self.h)
             self.h = None
 
@@ -159,6 +162,23 @@ class Plan:
 
     def pass_bytes(self, pass_index):
         return lib.vfd_plan_pass_bytes(self.h, int(pass_index))
+
+    def autotune(self, nfields=8, stream=None):
+        """Pick the fastest compiled schedule per pass; returns (variants, ms)."""
+        ms = (ctypes.c_float * 5)()
+        check(lib.vfd_plan_autotune(self.h, int(nfields), _stream(stream), ms))
+        return self.schedule, list(ms)
+
+    @property
+    def schedule(self):
+        v = (ctypes.c_int * 5)()
+        check(lib.vfd_plan_get_schedule(self.h, v))
+        return list(v)
+
+    @schedule.setter
+    def schedule(self, variants):
+        v = (ctypes.c_int * 5)(*variants)
+        check(lib.vfd_plan_set_schedule(self.h, v))
 
     def mass(self, tensors, stream=None):
         n = len(tensors)
