@@ -127,6 +127,68 @@ int vfd_field_mass(const vfd_plan* plan, const void* const* d_fields, int nfield
 int vfd_r2c_3d(const vfd_plan* plan, const void* d_real, void* d_complex, void* stream);
 int vfd_c2r_3d(const vfd_plan* plan, void* d_complex, void* d_real, void* stream);
 
+/* ======================================================================
+ * Engine: resident runs, batched ticks, serving.
+ *
+ * A run is one ABM instance: nfields fields of one grid (zero-padded linear
+ * convolution on the 2n box) with their Gaussian kernel spectra, resident in
+ * HBM for the run's lifetime.  Runs of every grid shape share per-shape plans
+ * and work-buffer pools.  Everything the engine allocates is charged against
+ * its HBM budget; a run that does not fit is refused with VFD_ERR_BUDGET.
+ * ====================================================================== */
+typedef struct vfd_engine vfd_engine;
+
+typedef struct {
+    int nz, ny, nx;        /* grid */
+    int nfields;           /* 1..32 */
+    int precision;         /* VFD_F32 or VFD_F64 */
+    const double* sigma2;  /* nfields Gaussian variances, sigma^2 = 2 D dt */
+    const double* decay;   /* nfields decay factors exp(-lambda dt) */
+} vfd_run_desc;
+
+typedef struct {
+    size_t budget_bytes, used_bytes;
+    int runs, shapes;
+    uint64_t batches, ticks, requests_done, requests_pending;
+} vfd_engine_stats;
+
+int vfd_engine_create(size_t hbm_budget_bytes, vfd_engine** engine);
+void vfd_engine_destroy(vfd_engine* engine);
+/* Device bytes a run of this description holds while resident (fields and
+ * spectra; work buffers are pooled per shape and charged when grown). */
+size_t vfd_run_bytes(const vfd_run_desc* desc);
+int vfd_run_create(vfd_engine* engine, const vfd_run_desc* desc, int* run_id);
+int vfd_run_destroy(vfd_engine* engine, int run_id);
+/* device pointer of a resident field (row-major [nz][ny][nx], precision of the run) */
+int vfd_run_field_ptr(vfd_engine* engine, int run_id, int field, void** d_field);
+/* Copy one field in / out (host or device memory, nz*ny*nx values of the run's
+ * precision).  Asynchronous on stream; host memory must stay valid until the
+ * stream reaches the copy. */
+int vfd_run_write_field(vfd_engine* engine, int run_id, int field, const void* src, void* stream);
+int vfd_run_read_field(vfd_engine* engine, int run_id, int field, void* dst, void* stream);
+
+/* Advance runs by `ticks` ticks: every field of every listed run, batched by
+ * grid shape (one launch per pass and shape, shapes on parallel branches).
+ * use_graph = 1 captures the tick into a hipGraph (cached per run set) and
+ * replays it.  Asynchronous on stream. */
+int vfd_step(vfd_engine* engine, const int* run_ids, int nruns, int ticks, int use_graph, void* stream);
+/* The same work one field at a time with per-pass launches and no graph
+ * (the plain per-field form, kept for comparison). */
+int vfd_step_plain(vfd_engine* engine, const int* run_ids, int nruns, int ticks, void* stream);
+
+/* Serving: a background loop takes one-tick requests for resident runs,
+ * admits them into the next batch as soon as an execution slot is free
+ * (continuous batching, `slots` batches in flight on separate streams), and
+ * records per-request latency from submission to result ready.  A run is in
+ * at most one in-flight batch; its requests run in submission order. */
+int vfd_serve_start(vfd_engine* engine, int slots, int max_batch_fields);
+int vfd_serve_stop(vfd_engine* engine);
+int vfd_submit(vfd_engine* engine, int run_id, uint64_t* ticket);
+/* done = 1 once the tick is complete; latency_ms from submission to completion */
+int vfd_poll(vfd_engine* engine, uint64_t ticket, int* done, double* latency_ms);
+int vfd_wait(vfd_engine* engine, uint64_t ticket, double* latency_ms);
+int vfd_engine_get_stats(vfd_engine* engine, vfd_engine_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

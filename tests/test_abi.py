@@ -1,0 +1,116 @@
+"""C-ABI checks that need no GPU: libvfd.so loads, exports every function the
+public headers declare, and rejects bad plans before touching the device.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "vf-version_7_3_amd", "libvfd.so")
+
+_DECL = re.compile(r"^\s*(?:extern\s+)?(?:const\s+)?[A-Za-z_][\w\s\*]*?[\s\*]([A-Za-z_]\w*)\s*\(", re.M)
+_SKIP = {"iDivUp", "iAlignUp", "if", "for", "while", "return", "sizeof"}
+
+
+def declared_functions():
+    names = {}
+    for fn in sorted(os.listdir(INCLUDE)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in _DECL.finditer(text):
+            name = m.group(1)
+            if name in _SKIP:
+                continue
+            names[name] = fn
+    return names
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libvfd.so is not built (run __graft_entry__.build())")
+    return ctypes.CDLL(LIB)
+
+
+def test_headers_declare_expected_entry_points():
+    names = declared_functions()
+    for must in ("fftDiffuse3D", "computeKernel3D", "computeKernelSpectrum3D", "computeKernel3DBatch",
+                 "snapTransformSize", "H2D", "D2H", "padDataConstantVF3D", "vfd_plan_create", "vfd_diffuse",
+                 "vfd_gauss_spectrum", "vfd_plan_autotune"):
+        assert must in names, must
+
+
+def test_library_exports_every_declared_function(lib):
+    exported = exported_symbols()
+    missing = [(n, h) for n, h in declared_functions().items() if n not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    assert "gpu_id" in exported      # extern int gpu_id[N_CHEM] (convolutionFFT_common.h:384)
+
+
+def test_snap_transform_size_matches_reference_rule(lib, orc):
+    lib.snapTransformSize.argtypes = [ctypes.c_int]
+    for n in list(range(1, 70)) + [100, 597, 1000, 1024, 1025, 1390, 2049, 5000]:
+        assert lib.snapTransformSize(n) == orc.snap_transform_size(n), n
+
+
+def test_length_support_table(lib):
+    lib.vfd_length_supported.argtypes = [ctypes.c_int]
+    for q in (2, 3, 4, 5, 6, 8, 9, 12, 16, 20, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 616, 768, 1024,
+              1152, 1280, 1536, 2048, 2304):
+        assert lib.vfd_length_supported(q) == 1, q
+    for q in (1, 7, 11, 13, 17, 100, 1000, 4096, 3072):
+        assert lib.vfd_length_supported(q) == 0, q
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("nz", "ny", "nx", "pz", "py", "px", "precision", "pad_mode",
+                                           "kspec_format")]
+
+
+@pytest.mark.parametrize("desc,code,needle", [
+    ((0, 4, 4, 0, 0, 0, 4, 100, 0), -1, "positive"),          # empty box
+    ((4, 4, 4, 2, 8, 8, 4, 100, 0), -1, "smaller"),           # FFT box smaller than data
+    ((4, 4, 5, 8, 8, 9, 4, 100, 0), -2, "even"),               # odd px
+    ((4, 4, 4, 8, 8, 8, 2, 100, 0), -1, "precision"),
+    ((4, 4, 4, 8, 8, 8, 4, 7, 0), -1, "pad mode"),
+    ((4, 4, 4, 8, 8, 8, 4, 100, 9), -1, "spectrum format"),
+    ((4, 4, 4, 8, 8, 34, 4, 100, 0), -2, "not compiled"),     # px/2 = 17
+    ((4, 4, 4, 8, 26, 8, 4, 100, 0), -2, "not compiled"),     # py = 26 = 2*13
+])
+def test_plan_validation_before_device(lib, desc, code, needle):
+    """Invalid dimensions are rejected with a code and a message, before any
+    device call (this container has no GPU)."""
+    lib.vfd_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_void_p)]
+    lib.vfd_last_error.restype = ctypes.c_char_p
+    h = ctypes.c_void_p()
+    rc = lib.vfd_plan_create(ctypes.byref(PlanDesc(*desc)), ctypes.byref(h))
+    assert rc == code
+    assert needle in lib.vfd_last_error().decode()
+    assert not h.value
+
+
+def test_null_arguments_are_errors(lib):
+    lib.vfd_plan_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.vfd_plan_create(None, None) == -1
+    lib.vfd_diffuse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    assert lib.vfd_diffuse(None, None, 0, None) == -1
+
+
+def test_legacy_bad_arguments_do_not_exit(lib):
+    """fftDiffuse3D with null buffers returns false instead of exiting."""
+    lib.fftDiffuse3D.restype = ctypes.c_bool
+    from vfd import legacy  # noqa: F401  (argtypes)
+    cc = legacy.CCtx()
+    assert not legacy.lib.fftDiffuse3D(None, None, 0, 0, cc, 1, 0.0, 0, 0)

@@ -29,7 +29,7 @@ static int blocks_per_cu(K kernel, int nt) {
     return cached;
 }
 
-static int num_cus() {
+[[maybe_unused]] static int num_cus() {
     static int ncu = 0;
     if (!ncu) {
         int dev = 0;
