@@ -15,6 +15,17 @@ between the launches on the launch stream over the timed region.  The CPU
 baseline is the oracle's FP32 OpenMP port timed on a bounded sample (one field
 of the same workload, one tick) on the host cores.
 
+Other workloads (one JSON line each, same fields where they apply):
+  --workload ensemble  64 resident runs of uneven grids drawn from
+                       {64x64x32, 128x128x64, 256x128x64, 256x256x64, 512x256x64},
+                       8 fields each: batched ticks replayed as a hipGraph vs the
+                       plain per-field form (one field at a time, per-pass
+                       launches, no graph).
+  --workload serve     Poisson request stream (mean 200/s) over resident runs
+                       drawn from that set plus 1024x512x128, admitted against a
+                       200 GB HBM budget; continuous batching; throughput and
+                       p50/p99 per-request latency.
+
 Prints ONE JSON line.
 """
 import argparse
@@ -32,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="128,512,1024", help="nz,ny,nx of the data box")
     ap.add_argument("--fields", type=int, default=8)
@@ -40,7 +51,167 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
+    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve"])
+    ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
+    ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
     return ap.parse_args()
+
+
+ENSEMBLE_GRIDS = [(32, 64, 64), (64, 128, 128), (64, 128, 256), (64, 256, 256), (64, 256, 512)]  # nz,ny,nx
+SERVE_GRIDS = ENSEMBLE_GRIDS + [(128, 512, 1024)]
+
+
+def _fill_run(eng, rid, shape, nf, seed):
+    import torch
+    fields, _ = synth_fields(shape, nf, seed, "cuda:0")
+    for f, t in enumerate(fields):
+        eng.write_field(rid, f, t)
+    torch.cuda.synchronize()
+
+
+def _run_params(rng, nf):
+    import numpy as np
+    D = rng.uniform(0.5, 5.0, nf)
+    lam = rng.uniform(0.001, 0.05, nf)
+    return list(2.0 * D), list(np.exp(-lam))
+
+
+def _header(args, metric, value, unit, ms_per_step, config, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (device-generated lognormal background + 50 point sources; "
+                    "random D, lambda per field)", "config": config}
+
+
+def bench_ensemble(args):
+    """Batched (graph) vs plain per-field ticks over uneven resident runs."""
+    import numpy as np
+    import torch
+    import vfd.engine as ve
+    rng = np.random.default_rng(args.seed)
+    nf = args.fields
+    picks = rng.integers(0, len(ENSEMBLE_GRIDS), args.runs)
+    eng = ve.Engine(int(250e9))
+    runs, vox = [], 0.0
+    t0 = time.perf_counter()
+    for i, g in enumerate(picks):
+        shape = ENSEMBLE_GRIDS[g]
+        s2, dc = _run_params(rng, nf)
+        rid = eng.create_run(shape, s2, dc)
+        _fill_run(eng, rid, shape, nf, args.seed + 7919 * i)
+        runs.append(rid)
+        vox += float(np.prod(shape)) * nf
+    setup_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / steps
+
+    K = args.steps
+    t_graph = timed(lambda: eng.step(runs, graph=True, stream=stream), K)
+    t_batch = timed(lambda: eng.step(runs, graph=False, stream=stream), K)
+    t_plain = timed(lambda: eng.step_plain(runs, stream=stream), max(1, K // 2))
+    counts = {f"{s[2]}x{s[1]}x{s[0]}": int((picks == i).sum()) for i, s in enumerate(ENSEMBLE_GRIDS)}
+    bytes_tick = 0.0
+    for i, s in enumerate(ENSEMBLE_GRIDS):
+        import vfd
+        p = vfd.Plan(s)
+        bytes_tick += p.info.bytes_per_field_tick * nf * counts[f"{s[2]}x{s[1]}x{s[0]}"]
+    out = _header(args, "voxel-updates/s, batched ensemble tick (64 uneven runs x 8 fields)", vox / t_graph,
+                  "voxel-updates/s", t_graph * 1e3,
+                  {"workload": f"{args.runs} resident runs x {nf} FP32 fields, grids drawn from "
+                               "{64x64x32,128x128x64,256x128x64,256x256x64,512x256x64}, one tick per step",
+                   "runs_per_grid": counts, "voxels_per_tick": vox})
+    out["batched_graph"] = {"ms_per_tick": t_graph * 1e3, "voxel_updates_s": vox / t_graph}
+    out["batched_eager"] = {"ms_per_tick": t_batch * 1e3, "voxel_updates_s": vox / t_batch}
+    out["plain_per_field"] = {"ms_per_tick": t_plain * 1e3, "voxel_updates_s": vox / t_plain}
+    out["speedup_vs_plain"] = t_plain / t_graph
+    out["algorithmic_gbs"] = bytes_tick / t_graph / 1e9
+    out["setup_s"] = setup_s
+    out["engine"] = eng.stats()
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Poisson request stream against resident runs; continuous batching."""
+    import threading
+    import numpy as np
+    import vfd
+    import vfd.engine as ve
+    rng = np.random.default_rng(args.seed)
+    nf = args.fields
+    budget = int(args.budget_gb * 1e9)
+    eng = ve.Engine(budget)
+    runs, vox_of, refused = [], {}, 0
+    for i in range(args.runs):
+        shape = SERVE_GRIDS[int(rng.integers(0, len(SERVE_GRIDS)))]
+        s2, dc = _run_params(rng, nf)
+        try:
+            rid = eng.create_run(shape, s2, dc)
+        except vfd.VfdError as e:
+            if e.code != vfd.VFD_ERR_BUDGET:
+                raise
+            refused += 1
+            continue
+        _fill_run(eng, rid, shape, nf, args.seed + 104729 * i)
+        runs.append(rid)
+        vox_of[rid] = float(np.prod(shape)) * nf
+    eng.serve_start(slots=2, max_batch_fields=256)
+    # warmup: one request per run (captures first-use costs)
+    for t in [eng.submit(r) for r in runs]:
+        eng.wait(t)
+    n_req = max(1, int(args.rate * args.duration))
+    gaps = rng.exponential(1.0 / args.rate, n_req)
+    which = rng.integers(0, len(runs), n_req)
+    tickets = [None] * n_req
+    lat = [0.0] * n_req
+    ready = threading.Semaphore(0)
+
+    def collect():
+        for k in range(n_req):
+            ready.acquire()
+            lat[k] = eng.wait(tickets[k])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for k in range(n_req):
+        due += gaps[k]
+        d = due - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        tickets[k] = eng.submit(runs[which[k]])
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.serve_stop()
+    vox = sum(vox_of[runs[w]] for w in which)
+    out = _header(args, "voxel-updates/s served (Poisson 200/s, continuous batching); p50/p99 latency",
+                  vox / wall, "voxel-updates/s", wall * 1e3 / n_req,
+                  {"workload": f"Poisson stream mean {args.rate}/s for {args.duration}s over resident runs of "
+                               f"{nf} FP32 fields, grids from the ensemble set plus 1024x512x128, "
+                               f"HBM budget {args.budget_gb} GB",
+                   "runs_admitted": len(runs), "runs_refused": refused})
+    out["requests"] = n_req
+    out["requests_per_s"] = n_req / wall
+    out["latency_ms"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                         "mean": float(np.mean(lat)), "max": float(np.max(lat))}
+    out["engine"] = st
+    eng.close()
+    print(json.dumps(out), flush=True)
 
 
 def synth_fields(shape, nfields, seed, device):
@@ -92,6 +263,11 @@ def main():
     import torch
     import vfd
 
+    torch.cuda.set_device(0)
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     shape = tuple(int(s) for s in args.shape.split(","))
     nz, ny, nx = shape
     dev = "cuda:0"

@@ -130,6 +130,28 @@ bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlan
                   int iter);
 #endif
 
+/* ---- batched per-tick form of the chem loop of Diffuser::diffuseChemGPU
+ * (Diffuser.cpp:300-421): for ic < nchem, h_ibuffs[ic] -> d_data[ic] -> the
+ * fftDiffuse3D operator (constant-VF pad with baseChem[ic] and epiBoundary,
+ * spectrum d_kernelspectrum_h[ic]) -> h_obuffs[ic].  Copies in, diffusion and
+ * copies out run on three streams, so one chem's copies overlap its
+ * neighbours' diffusion; distinct d_data[ic] buffers give the most overlap.
+ * Synchronous on return; false (and vfd_last_error) on bad arguments. ---- */
+bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const float* baseChem);
+
+/* ---- memory for the diffusion_helper / Diffuser layer and WHChemical: stands
+ * in for the cudaGetDeviceCount / cudaSetDevice / cudaMalloc / cudaFree /
+ * cudaMallocHost / cudaFreeHost calls of DiffusionHelper.cpp:6-26, 160-225,
+ * 464-588 and WHChemical.cpp:198-206.  Pointer functions return NULL on
+ * failure; int functions return 0 or a negative vfd error code. ---- */
+int vfdDeviceCount(void);
+int vfdSetDevice(int dev);
+void* vfdMalloc(size_t bytes);
+void vfdFree(void* d_ptr);
+void* vfdHostAlloc(size_t bytes); /* pinned (page-locked) host memory */
+void vfdHostFree(void* h_ptr);
+int vfdMemcpy(void* dst, const void* src, size_t bytes); /* any direction, synchronous */
+
 /* ---- plan handles standing in for cufftPlan3d / cufftDestroy (Diffuser.cpp:273-274,
  * 429-430).  type is VFD_R2C or VFD_C2R; a plan only records the box. ---- */
 int vfdPlan3d(cufftHandle* plan, int nz, int ny, int nx, int type);

@@ -16,11 +16,14 @@ _DECL = re.compile(r"^\s*(?:extern\s+)?(?:const\s+)?[A-Za-z_][\w\s\*]*?[\s\*]([A
 _SKIP = {"iDivUp", "iAlignUp", "if", "for", "while", "return", "sizeof"}
 
 
-def declared_functions():
+C_HEADERS = ("vfd.h", "vf_diffusion.h")          # the extern "C" interface of libvfd.so
+CXX_HEADERS = ("DiffusionHelper.h", "Diffuser.h")  # C++ surface compiled into the ABM's build
+CXX_CHECK = os.path.join(ROOT, "tests", "cxx", "libdiffuser_check.so")
+
+
+def declared_functions(headers=C_HEADERS):
     names = {}
-    for fn in sorted(os.listdir(INCLUDE)):
-        if not fn.endswith(".h"):
-            continue
+    for fn in headers:
         text = open(os.path.join(INCLUDE, fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
@@ -57,6 +60,24 @@ def test_library_exports_every_declared_function(lib):
     missing = [(n, h) for n, h in declared_functions().items() if n not in exported]
     assert not missing, f"declared but not exported: {missing}"
     assert "gpu_id" in exported      # extern int gpu_id[N_CHEM] (convolutionFFT_common.h:384)
+
+
+def test_cxx_surface_builds_and_exports_reference_names():
+    """diffusion_helper::* and Diffuser (DiffusionHelper.h:30-43, Diffuser.h:24-70)
+    compile against the ABM-side header set and export every declared name."""
+    if not os.path.exists(CXX_CHECK):
+        pytest.fail("tests/cxx/libdiffuser_check.so is not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-DC", "--defined-only", CXX_CHECK], capture_output=True, text=True,
+                         check=True).stdout
+    for name in ("findGPUs", "initKernelDimensions", "allocKernelBuffers", "deallocConvCtxBuffers",
+                 "initChemDimensions", "allocChemBuffers", "prepAndComputeKernelSpectrum", "prepKernel",
+                 "printContext"):
+        assert f"diffusion_helper::{name}(" in out, name
+    for sig in ("Diffuser::Diffuser()", "Diffuser::Diffuser(int, CCTX*, CCTX*, WHChemical*)",
+                "Diffuser::diffuseChemGPU(int)", "Diffuser::~Diffuser()"):
+        assert sig in out, sig
+    names = declared_functions(CXX_HEADERS)
+    assert {"initKernelDimensions", "allocChemBuffers", "diffuseChemGPU"} <= set(names)
 
 
 def test_snap_transform_size_matches_reference_rule(lib, orc):

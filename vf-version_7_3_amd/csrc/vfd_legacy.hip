@@ -291,6 +291,41 @@ int vfdDestroy(cufftHandle plan) {
     return g_plans.erase(plan) ? VFD_OK : set_error(VFD_ERR_NOTFOUND, "vfdDestroy: unknown plan %d", plan);
 }
 
+// memory helpers behind the diffusion_helper layer (DiffusionHelper.cpp)
+int vfdDeviceCount(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int vfdSetDevice(int dev) { return hip_check(hipSetDevice(dev), "vfdSetDevice"); }
+
+void* vfdMalloc(size_t bytes) {
+    void* p = nullptr;
+    return hip_check(hipMalloc(&p, bytes), "vfdMalloc") == VFD_OK ? p : nullptr;
+}
+
+void vfdFree(void* p) {
+    if (p) (void)hip_check(hipFree(p), "vfdFree");
+}
+
+void* vfdHostAlloc(size_t bytes) {
+    void* p = nullptr;
+    return hip_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "vfdHostAlloc") == VFD_OK ? p : nullptr;
+}
+
+void vfdHostFree(void* p) {
+    if (p) (void)hip_check(hipHostFree(p), "vfdHostFree");
+}
+
+int vfdMemcpy(void* dst, const void* src, size_t bytes) {
+    if ((!dst || !src) && bytes) return set_error(VFD_ERR_INVALID, "vfdMemcpy: null pointer");
+    return hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDefault), "vfdMemcpy");
+}
+
 // H2D / D2H (Diffusion.cu:72-122; under M40 the spectrum stays on the device)
 void H2D(int ic, c_ctx* cc, int np) {
     if (!cc || ic < 0 || ic >= N_CHEM || np < 0) {
@@ -354,6 +389,90 @@ bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlan
     vfd_field f{d_Data, d_Data, it->second.work, d_KernelSpectrum, (double)baseChem, epiBoundary, 0};
     if (int rc = plan_diffuse(it->second.plan, &f, 1, nullptr)) return fail(rc);
     return hip_ok(hipStreamSynchronize(nullptr), "fftDiffuse3D");
+}
+
+// Batched, pipelined form of the per-chem loop of Diffuser::diffuseChemGPU
+// (Diffuser.cpp:300-421, which runs H2D -> fftDiffuse3D -> D2H for one chem at
+// a time with a device synchronization after every step).  Here chem ic's copy
+// in runs on one stream, its diffusion (the fftDiffuse3D operator) on a second
+// and its copy out on a third, ordered by events, so the copies of one chem
+// overlap the diffusion of its neighbours.  A device buffer d_data[ic] shared
+// by several chems is reused only after the previous chem's copy out.
+namespace {
+struct Pipe {
+    hipStream_t in = nullptr, comp = nullptr, out = nullptr;
+    hipEvent_t e_in[N_CHEM] = {}, e_comp[N_CHEM] = {}, e_out[N_CHEM] = {};
+};
+std::map<int, Pipe> g_pipes;
+
+int pipe_for(int dev, Pipe** out) {
+    auto it = g_pipes.find(dev);
+    if (it != g_pipes.end()) {
+        *out = &it->second;
+        return VFD_OK;
+    }
+    Pipe p;
+    for (hipStream_t* s : {&p.in, &p.comp, &p.out})
+        if (int rc = hip_check(hipStreamCreateWithFlags(s, hipStreamNonBlocking), "chem pipeline stream")) return rc;
+    for (int i = 0; i < N_CHEM; ++i)
+        for (hipEvent_t* e : {&p.e_in[i], &p.e_comp[i], &p.e_out[i]})
+            if (int rc = hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "chem pipeline event")) return rc;
+    *out = &g_pipes.emplace(dev, p).first->second;
+    return VFD_OK;
+}
+}  // namespace
+
+extern "C" bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const float* baseChem) {
+    if (nchem < 0 || nchem > N_CHEM || (nchem > 0 && !baseChem))
+        return fail(set_error(VFD_ERR_INVALID, "vfdDiffuseChems: nchem must be 0..%d with baseChem given", N_CHEM));
+    if (cctx.DD < 1 || cctx.DH < 1 || cctx.DW < 1)
+        return fail(set_error(VFD_ERR_INVALID, "vfdDiffuseChems: empty data box"));
+    for (int ic = 0; ic < nchem; ++ic)
+        if (!cctx.d_data[ic] || !cctx.d_kernelspectrum_h[ic] || !cctx.h_ibuffs[ic] || !cctx.h_obuffs[ic])
+            return fail(set_error(VFD_ERR_INVALID, "vfdDiffuseChems: chem %d has a null buffer", ic));
+    if (nchem == 0) return true;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+    const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
+    auto it = g_ws.find(key);
+    if (it == g_ws.end()) {
+        vfd_plan_desc d{cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW, VFD_F32, VFD_PAD_CONSTANT_VF,
+                        VFD_KSPEC_COMPLEX_STD};
+        Ws ws;
+        if (int rc = plan_init(&d, &ws.plan)) return fail(rc);
+        if (!hip_ok(hipMalloc(&ws.work, ws.plan.work_bytes), "vfdDiffuseChems workspace")) return false;
+        it = g_ws.emplace(key, ws).first;
+    }
+    Pipe* pp = nullptr;
+    if (int rc = pipe_for(dev, &pp)) return fail(rc);
+    const size_t bytes = (size_t)cctx.DD * cctx.DH * cctx.DW * sizeof(float);
+    int rc = VFD_OK;
+    for (int ic = 0; ic < nchem && !rc; ++ic) {
+        float* d = cctx.d_data[ic];
+        for (int jc = ic - 1; jc >= 0 && !rc; --jc)
+            if (cctx.d_data[jc] == d) {  // reuse only after that chem's copy out
+                rc = hip_check(hipStreamWaitEvent(pp->in, pp->e_out[jc], 0), "pipeline wait");
+                break;
+            }
+        if (!rc) rc = hip_check(hipMemcpyAsync(d, cctx.h_ibuffs[ic], bytes, hipMemcpyHostToDevice, pp->in), "H2D");
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_in[ic], pp->in), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_in[ic], 0), "pipeline wait");
+        if (!rc) {
+            vfd_field f{d, d, it->second.work, cctx.d_kernelspectrum_h[ic], (double)baseChem[ic], epiBoundary, 0};
+            rc = plan_diffuse(it->second.plan, &f, 1, pp->comp);
+        }
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_comp[ic], pp->comp), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_comp[ic], 0), "pipeline wait");
+        if (!rc) rc = hip_check(hipMemcpyAsync(cctx.h_obuffs[ic], d, bytes, hipMemcpyDeviceToHost, pp->out), "D2H");
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_out[ic], pp->out), "pipeline event");
+    }
+    for (hipStream_t s : {pp->in, pp->comp, pp->out}) {
+        const int src = hip_check(hipStreamSynchronize(s), "pipeline sync");
+        if (!rc) rc = src;
+    }
+    return rc ? fail(rc) : true;
 }
 
 // --------------------------------------------------------- kernel build
