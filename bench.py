@@ -21,6 +21,7 @@ Other workloads (one JSON line each, same fields where they apply):
                        8 fields each: batched ticks replayed as a hipGraph vs the
                        plain per-field form (one field at a time, per-pass
                        launches, no graph).
+  --workload copy      1 GiB device-to-device copy: measured HBM bandwidth.
   --workload serve     Poisson request stream (mean 200/s) over resident runs
                        drawn from that set plus 1024x512x128, admitted against a
                        200 GB HBM budget; continuous batching; throughput and
@@ -51,7 +52,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
-    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve"])
+    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve", "copy"])
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
@@ -84,6 +85,31 @@ def _header(args, metric, value, unit, ms_per_step, config, dtype="f32"):
             "scaling": "weak", "vs_baseline": None, "dtype": dtype,
             "data": "synthetic (device-generated lognormal background + 50 point sources; "
                     "random D, lambda per field)", "config": config}
+
+
+def bench_copy(args):
+    """Device-to-device copy of 1 GiB (torch): the measured HBM bandwidth that
+    sits beside the 8 TB/s spec peak, and the known-size kernel the HBM
+    counters (FETCH_SIZE, WRITE_SIZE) are calibrated on."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.rand(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    gbs = 2.0 * n * 4 / (ms * 1e-3) / 1e9
+    out = _header(args, "HBM copy bandwidth (1 GiB device-to-device, read + write bytes)", gbs, "GB/s", ms,
+                  {"workload": "torch copy_ of a 1 GiB FP32 tensor"})
+    out["frac_of_spec"] = gbs / HBM_PEAK_GBS
+    print(json.dumps(out), flush=True)
 
 
 def bench_ensemble(args):
@@ -268,6 +294,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     shape = tuple(int(s) for s in args.shape.split(","))
     nz, ny, nx = shape
     dev = "cuda:0"

@@ -51,6 +51,7 @@ def lib():
         L.orc_gauss_radius.argtypes = [_d]
         L.orc_gauss_radius.restype = _i
         L.orc_direct_separable3d.argtypes = [_p, _i, _i, _i, _d, _d, _p]
+        L.orc_direct_separable3d_f64.argtypes = [_p, _i, _i, _i, _d, _d, _p]
         L.orc_gauss_window3d.argtypes = [_d, _d, _p, _i]
         L.orc_compute_kernel3d.argtypes = [_i, _d, _d, _d, _i] + [_i] * 9 + [_i, _p]
         L.orc_stencil_heat_window.argtypes = [_d, _d, _d, _i, _i, _i, _i, _i, _p]
@@ -197,9 +198,16 @@ def gauss_window3d(sigma2, decay, R=None):
 
 
 def direct_separable(u, sigma2, decay):
-    u = np.ascontiguousarray(u, dtype=np.float32)
+    """Separable direct Gaussian convolution in FP64.  An FP32 input is the
+    reference's field type; an FP64 input (a previous FP64 tick) is used as is."""
+    if np.asarray(u).dtype == np.float64:
+        u = np.ascontiguousarray(u)
+        fn = lib().orc_direct_separable3d_f64
+    else:
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        fn = lib().orc_direct_separable3d
     out = np.empty(u.shape, dtype=np.float64)
-    assert lib().orc_direct_separable3d(_ptr(u), *u.shape, float(sigma2), float(decay), _ptr(out)) == 0
+    assert fn(_ptr(u), *u.shape, float(sigma2), float(decay), _ptr(out)) == 0
     return out
 
 

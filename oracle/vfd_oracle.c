@@ -15,7 +15,7 @@
  *   - the diffusion operator is pad -> R2C -> (x K^ / (FD*FH*FW)) -> C2R -> unpad
  *     (Diffusion.cu:1778-1868, convolutionFFT.cu:578).
  *
- * Pinning (how each function is checked; see tests/test_oracle_*.py):
+ * Pinning (how each function is checked; see tests/test_oracle.py):
  *   orc_fft_c2c / orc_fft_c2c_f32 : O(n^2) DFT on every n <= 64, numpy golden vectors,
  *                                   impulse/cosine/Parseval/round-trip identities.
  *   orc_rfft3 / orc_irfft3        : numpy rfftn/irfftn golden fixtures (tests/golden),
@@ -385,8 +385,25 @@ int orc_gauss_radius(double sigma2) { return (int)ceil(6.0 * sqrt(sigma2)); }
 
 /* Separable direct zero-padded linear convolution with the Gaussian window and
  * the decay factor, evaluated on the data box.  Independent of every FFT. */
+static int separable_core(const float *uf, const double *ud, int D, int H, int W, double sigma2,
+                          double decay, double *out);
+
 int orc_direct_separable3d(const float *u, int D, int H, int W, double sigma2, double decay,
                            double *out)
+{
+    return separable_core(u, NULL, D, H, W, sigma2, decay, out);
+}
+
+/* The same with an FP64 input field: chains ticks of the FP64 path without
+ * rounding the intermediate field to FP32. */
+int orc_direct_separable3d_f64(const double *u, int D, int H, int W, double sigma2, double decay,
+                               double *out)
+{
+    return separable_core(NULL, u, D, H, W, sigma2, decay, out);
+}
+
+static int separable_core(const float *uf, const double *ud, int D, int H, int W, double sigma2,
+                          double decay, double *out)
 {
     int R = orc_gauss_radius(sigma2);
     double *g = (double *)malloc(sizeof(double) * (2 * R + 1));
@@ -395,7 +412,7 @@ int orc_direct_separable3d(const float *u, int D, int H, int W, double sigma2, d
     double *b = (double *)malloc(sizeof(double) * n);
     if (!g || !a || !b) { free(g); free(a); free(b); return -2; }
     orc_gauss_window1d(sigma2, R, g);
-    for (long i = 0; i < n; i++) a[i] = u[i];
+    for (long i = 0; i < n; i++) a[i] = uf ? (double)uf[i] : ud[i];
     /* x */
 #pragma omp parallel for schedule(static)
     for (long r = 0; r < (long)D * H; r++)

@@ -45,7 +45,6 @@ def test_diffuser_ticks_match_oracle(check_lib, orc, shape, dt, ticks):
     gam = rng.uniform(1e-5, 1e-4, 8).astype(np.float32)       # decay rate per second
     base = rng.uniform(0.0, 0.5, 8).astype(np.float32)
     wall = nx - 3
-    n = nx * ny * nz
     u0 = np.stack([orc.synth_field(shape, 300 + ic, nsrc=4) for ic in range(8)]).astype(np.float32)
     fields = np.ascontiguousarray(u0.copy())
     info = (ctypes.c_int * 13)()

@@ -253,3 +253,17 @@ def test_synth_deterministic(orc):
     assert not np.array_equal(a, orc.synth_field((4, 5, 6), 43))
     D, lam = orc.synth_params(42, 3)
     assert 0.5 <= D <= 5.0 and 0.001 <= lam <= 0.05
+
+
+def test_separable_f64_input_matches_f32_entry(orc):
+    """The FP64-input separable entry point (used to chain FP64 ticks) equals
+    the FP32-input one on FP32-representable data, and keeps an FP64 field's
+    low bits (an FP32 round trip of the input changes the result)."""
+    u = orc.synth_field((6, 7, 9), 17, nsrc=3)
+    a = orc.direct_separable(u, 1.3, 0.97)
+    b = orc.direct_separable(u.astype(np.float64), 1.3, 0.97)
+    assert np.array_equal(a, b)
+    r1 = orc.direct_separable(a, 1.3, 0.97)
+    r1f = orc.direct_separable(a.astype(np.float32), 1.3, 0.97)
+    assert not np.array_equal(r1, r1f)
+    assert orc.rel_l2(r1f, r1) < 1e-7
