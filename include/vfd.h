@@ -46,8 +46,14 @@ enum { VFD_PAD_CONSTANT_VF = 3, VFD_PAD_ZERO = 100 };
 /* kernel-spectrum formats:
  *  VFD_KSPEC_REAL        real K^ of an even kernel, [pz][py][ld] (vfd_plan_info ld)
  *  VFD_KSPEC_COMPLEX_STD complex K^ in the R2C layout [pz][py][px/2+1] of the
- *                        reference (d_kernelspectrum_h, Diffusion.cu:1605) */
-enum { VFD_KSPEC_REAL = 0, VFD_KSPEC_COMPLEX_STD = 1 };
+ *                        reference (d_kernelspectrum_h, Diffusion.cu:1605)
+ *  VFD_KSPEC_SEPARABLE   a separable even kernel as FP64 tables
+ *                        [scale | gx[px/2+1] | gy[py] | gz[pz]]; the z pass forms
+ *                        K^(kz,ky,kx) = ((gx*gy)*scale)*gz in FP64 and rounds it
+ *                        to the plan's precision, the same numbers VFD_KSPEC_REAL
+ *                        stores for the Gaussian, without streaming a 3D array.
+ *                        VFD_PAD_ZERO plans only. */
+enum { VFD_KSPEC_REAL = 0, VFD_KSPEC_COMPLEX_STD = 1, VFD_KSPEC_SEPARABLE = 2 };
 
 typedef struct vfd_plan vfd_plan;
 
@@ -56,7 +62,7 @@ typedef struct {
     int pz, py, px;  /* FFT box; 0 means 2*n (zero-padded linear convolution) */
     int precision;   /* VFD_F32 or VFD_F64 */
     int pad_mode;    /* VFD_PAD_ZERO or VFD_PAD_CONSTANT_VF */
-    int kspec_format;/* VFD_KSPEC_REAL or VFD_KSPEC_COMPLEX_STD */
+    int kspec_format;/* VFD_KSPEC_REAL, VFD_KSPEC_COMPLEX_STD or VFD_KSPEC_SEPARABLE */
 } vfd_plan_desc;
 
 typedef struct {
@@ -110,9 +116,11 @@ int vfd_plan_autotune(vfd_plan* plan, int nfields, void* stream, float* best_ms)
 int vfd_plan_get_schedule(const vfd_plan* plan, int* variants);  /* 5 ints */
 int vfd_plan_set_schedule(vfd_plan* plan, const int* variants);  /* 5 ints, 0..3 */
 
-/* Real kernel spectrum (VFD_KSPEC_REAL) of the normalized sampled Gaussian
- * g(j) ~ exp(-j^2/(2 sigma2)), |j| <= ceil(6 sqrt(sigma2)) per axis, times decay,
- * computed in FP64 and stored in the plan's precision. */
+/* Kernel spectrum of the normalized sampled Gaussian g(j) ~ exp(-j^2/(2 sigma2)),
+ * |j| <= ceil(6 sqrt(sigma2)) per axis, times decay, with 1/(px py pz) folded
+ * in: the 3D array in the plan's precision (VFD_KSPEC_REAL) or the FP64 tables
+ * (VFD_KSPEC_SEPARABLE), computed in FP64.  Both formats give bit-identical
+ * ticks. */
 int vfd_gauss_spectrum(const vfd_plan* plan, double sigma2, double decay, void* d_kspec, void* stream);
 
 /* Compensated (Neumaier, FP64) sum of each of nfields real fields of the plan's
@@ -131,7 +139,8 @@ int vfd_c2r_3d(const vfd_plan* plan, void* d_complex, void* d_real, void* stream
  * Engine: resident runs, batched ticks, serving.
  *
  * A run is one ABM instance: nfields fields of one grid (zero-padded linear
- * convolution on the 2n box) with their Gaussian kernel spectra, resident in
+ * convolution on the 2n box) with their Gaussian kernels (separable FP64
+ * spectrum tables, VFD_KSPEC_SEPARABLE), resident in
  * HBM for the run's lifetime.  Runs of every grid shape share per-shape plans
  * and work-buffer pools.  Everything the engine allocates is charged against
  * its HBM budget; a run that does not fit is refused with VFD_ERR_BUDGET.

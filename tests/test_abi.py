@@ -109,6 +109,7 @@ class PlanDesc(ctypes.Structure):
     ((4, 4, 4, 8, 8, 8, 4, 100, 9), -1, "spectrum format"),
     ((4, 4, 4, 8, 8, 34, 4, 100, 0), -2, "not compiled"),     # px/2 = 17
     ((4, 4, 4, 8, 26, 8, 4, 100, 0), -2, "not compiled"),     # py = 26 = 2*13
+    ((4, 4, 4, 8, 8, 8, 4, 3, 2), -1, "separable"),           # separable spectrum needs the zero pad
 ])
 def test_plan_validation_before_device(lib, desc, code, needle):
     """Invalid dimensions are rejected with a code and a message, before any

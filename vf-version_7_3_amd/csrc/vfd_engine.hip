@@ -146,7 +146,7 @@ namespace {
 size_t run_bytes_for(const Plan& p, int nfields) { return (size_t)nfields * (p.field_bytes + p.kspec_bytes); }
 
 int make_plan_for(const ShapeKey& k, Plan* p) {
-    vfd_plan_desc d{k.nz, k.ny, k.nx, 0, 0, 0, k.prec, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    vfd_plan_desc d{k.nz, k.ny, k.nx, 0, 0, 0, k.prec, VFD_PAD_ZERO, VFD_KSPEC_SEPARABLE};
     return plan_init(&d, p);
 }
 
@@ -375,7 +375,7 @@ void vfd_engine_destroy(vfd_engine* e) {
 size_t vfd_run_bytes(const vfd_run_desc* d) {
     if (!d) return 0;
     Plan p;
-    vfd_plan_desc pd{d->nz, d->ny, d->nx, 0, 0, 0, d->precision, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    vfd_plan_desc pd{d->nz, d->ny, d->nx, 0, 0, 0, d->precision, VFD_PAD_ZERO, VFD_KSPEC_SEPARABLE};
     if (plan_init(&pd, &p)) return 0;
     return run_bytes_for(p, d->nfields);
 }

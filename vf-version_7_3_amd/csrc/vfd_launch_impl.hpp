@@ -102,14 +102,24 @@ hipError_t launch_xinv<VFD_REAL>(int L, const FieldBatch& fields, int nfields, c
 #endif
 
 #ifdef VFD_COL_TU
-template <typename T, int Q, int MODE, int RS, int V>
+template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                          const void* tw, hipStream_t st) {
     using C = ColCfg<T, Q, V>;
     dim3 grid((unsigned)((cd.ncols + C::TC - 1) / C::TC), (unsigned)nplanes, (unsigned)nfields);
-    hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V>), grid, dim3(C::NT), 0, st, fields, cd,
+    hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), grid, dim3(C::NT), 0, st, fields, cd,
                        (const cx<T>*)tw);
     return hipGetLastError();
+}
+
+// fused pass: the spectrum format picks the instantiation (the R2C-layout
+// complex format of the reference interface keeps the default schedule)
+template <typename T, int Q, int RS, int V>
+static hipError_t col_fused(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
+                            const void* tw, hipStream_t st) {
+    if (cd.kspec_sep) return col_go<T, Q, COL_FUSED, RS, V, KS_SEP>(fields, nfields, cd, nplanes, tw, st);
+    if (cd.kspec_complex) return col_go<T, Q, COL_FUSED, RS, 0, KS_COMPLEX>(fields, nfields, cd, nplanes, tw, st);
+    return col_go<T, Q, COL_FUSED, RS, V, KS_REAL>(fields, nfields, cd, nplanes, tw, st);
 }
 
 template <typename T, int Q, int V>
@@ -121,8 +131,8 @@ static hipError_t col_mode(const FieldBatch& fields, int nfields, const ColDesc&
         case COL_FWD * 2 + 1: return col_go<T, Q, COL_FWD, 2, V>(fields, nfields, cd, nplanes, tw, st);
         case COL_INV * 2 + 0: return col_go<T, Q, COL_INV, 1, V>(fields, nfields, cd, nplanes, tw, st);
         case COL_INV * 2 + 1: return col_go<T, Q, COL_INV, 2, V>(fields, nfields, cd, nplanes, tw, st);
-        case COL_FUSED * 2 + 0: return col_go<T, Q, COL_FUSED, 1, V>(fields, nfields, cd, nplanes, tw, st);
-        case COL_FUSED * 2 + 1: return col_go<T, Q, COL_FUSED, 2, V>(fields, nfields, cd, nplanes, tw, st);
+        case COL_FUSED * 2 + 0: return col_fused<T, Q, 1, V>(fields, nfields, cd, nplanes, tw, st);
+        case COL_FUSED * 2 + 1: return col_fused<T, Q, 2, V>(fields, nfields, cd, nplanes, tw, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -45,9 +45,14 @@ template <typename T, int Q, int V = 0> struct ColCfg {
     static constexpr int MAXR = var_maxr(V);
     static constexpr int TPF = tpf_for(Q, MAXR);
     static constexpr int LDS_CAP = 72 * 1024;
+    // columns per block: the variant's thread cap, but never narrower than one
+    // 128-byte line of a row (narrower tiles fetch whole lines and use part)
+    static constexpr int TC_LINE = 128 / (int)sizeof(cx<T>);
+    static constexpr bool lds_fits(int tc) { return (tc * Q + 2 * Q) * (int)sizeof(cx<T>) <= LDS_CAP; }
     static constexpr int tc_for() {
         int tc = 64;
-        while (tc > 1 && (tc * TPF > var_ntmax(V) || (tc * Q + 2 * Q) * (int)sizeof(cx<T>) > LDS_CAP)) tc /= 2;
+        while (tc > 1 && (tc * TPF > var_ntmax(V) || !lds_fits(tc))) tc /= 2;
+        while (tc < TC_LINE && lds_fits(2 * tc) && 2 * tc * TPF <= 1024) tc *= 2;
         return tc;
     }
     static constexpr int TC = tc_for();  // columns per block
@@ -240,7 +245,11 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
 //   COL_FUSED: nin rows in, forward, x K^[k]*scale, inverse, nout rows out.
 //              The inverse runs the reversed radix plan, so it starts on the
 //              elements the forward left in registers (no LDS redistribution).
-template <typename T, int Q, int MODE, int RS, int V>
+// KS (fused mode): the kernel-spectrum format -- KS_REAL, KS_COMPLEX (R2C
+// layout, scaled by cd.scale), or KS_SEP (FP64 tables, K^ formed here).
+enum { KS_REAL = 0, KS_COMPLEX = 1, KS_SEP = 2 };
+
+template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 __global__ void __launch_bounds__((ColCfg<T, Q, V>::NT))
 col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     using C = ColCfg<T, Q, V>;
@@ -251,6 +260,8 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     constexpr unsigned CB = sizeof(cx<T>), RB = sizeof(T);
     __shared__ cx<T> smem[C::TC * Q];
     __shared__ cx<T> tw[C::TW_N];
+    constexpr bool SEP = MODE == COL_FUSED && KS == KS_SEP;
+    __shared__ double gzs[SEP ? P : 1];  // separable: the gz table of the z frequencies
     const FieldDesc& fd = fb.f[blockIdx.z];
     const rsrc_t rwork = make_rsrc(fd.work, cd.work_bytes);
     const int c = threadIdx.x % C::TC;
@@ -300,11 +311,23 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     // kernel spectrum for the fused mode, prefetched before any transform work:
     // real K^ stores the RS frequencies m*RS + s of one element side by side
     // (one load per element), complex K^ is the R2C layout (one load per s)
-    [[maybe_unused]] cx<T> kv[MODE == COL_FUSED ? NK : 1];
-    [[maybe_unused]] cx<T> kvc[MODE == COL_FUSED && RS > 1 ? NK : 1];
-    if constexpr (MODE == COL_FUSED) {
+    constexpr bool STREAM_K = MODE == COL_FUSED && KS != KS_SEP;
+    [[maybe_unused]] cx<T> kv[STREAM_K ? NK : 1];
+    [[maybe_unused]] cx<T> kvc[STREAM_K && KS == KS_COMPLEX && RS > 1 ? NK : 1];
+    [[maybe_unused]] double ka = 0.0;  // separable: (gx*gy)*scale of this column
+    if constexpr (SEP) {
+        // separable K^(kz) = (((gx*gy)*scale)*gz[kz]) rounded to T: the numbers
+        // the 3D real layout stores (gauss_product_kernel); gz staged in LDS
+        using BD = BufOps<double>;
         const rsrc_t rk = make_rsrc(fd.kspec, cd.kspec_bytes);
-        if (cd.kspec_complex) {
+        const double sc = BD::ld_r(rk, 0u);
+        const double gyv = BD::ld_r(rk, (unsigned)(cd.sep_gy + blockIdx.y) * 8u);
+        const double gxv = BD::ld_r(rk, live ? (unsigned)(1 + col) * 8u : kBufOOB);
+        ka = (gxv * gyv) * sc;
+        for (int i = threadIdx.x; i < P; i += C::NT) gzs[i] = BD::ld_r(rk, (unsigned)(cd.sep_gz + i) * 8u);
+    } else if constexpr (MODE == COL_FUSED) {
+        const rsrc_t rk = make_rsrc(fd.kspec, cd.kspec_bytes);
+        if constexpr (KS == KS_COMPLEX) {
             const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col) * CB;
             const unsigned kstep = (unsigned)cd.ks_row * CB;
 #pragma unroll
@@ -318,6 +341,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                         kvc[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (unsigned)(m * RS + 1) * kstep : kBufOOB);
                 }
         } else {
+            static_assert(KS == KS_REAL, "streamed spectrum format");
             const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col * RS) * RB;
             const unsigned kstep = (unsigned)cd.ks_row * RB;
 #pragma unroll
@@ -350,7 +374,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                     v[b * ShF::R0 + r] = val;
                 }
             fft_regs<T, Q, C::TPF, false, false, C::MAXR>(t, v, lds, tw, RS);
-            if (cd.kspec_complex) {
+            if constexpr (KS == KS_COMPLEX) {
                 const T sc = (T)cd.scale;
 #pragma unroll
                 for (int i = 0; i < NK; ++i) {
@@ -358,6 +382,15 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                     if constexpr (RS > 1) k = s ? kvc[i] : kv[i];
                     v[i] = cscale(cmul(v[i], k), sc);
                 }
+            } else if constexpr (KS == KS_SEP) {
+#pragma unroll
+                for (int b = 0; b < ShF::NBL; ++b)
+#pragma unroll
+                    for (int r = 0; r < ShF::RL; ++r) {
+                        const int m = ShF::out_elem(t, b, r);
+                        const T k = (T)(ka * gzs[m >= 0 ? m * RS + s : 0]);
+                        v[b * ShF::RL + r] = cscale(v[b * ShF::RL + r], k);
+                    }
             } else {
 #pragma unroll
                 for (int i = 0; i < NK; ++i) v[i] = cscale(v[i], s ? kv[i].y : kv[i].x);

@@ -49,6 +49,8 @@ struct ColDesc {
     long ks_row;        // kernel-spectrum stride along this axis' frequency
     long ks_plane;      // kernel-spectrum stride along the plane index
     int kspec_complex;
+    int kspec_sep;      // K^ from FP64 tables [scale | gx | gy | gz]
+    int sep_gy, sep_gz; // table offsets (elements) of gy and gz
     double scale;       // applied to K^ products (1 / (Px Py Pz))
     unsigned long work_bytes, kspec_bytes;  // per-field array sizes
 };

@@ -16,7 +16,7 @@ VFD_ERR_INVALID, VFD_ERR_UNSUPPORTED, VFD_ERR_DEVICE = -1, -2, -3
 VFD_ERR_NOMEM, VFD_ERR_BUDGET, VFD_ERR_NOTFOUND, VFD_ERR_STATE = -4, -5, -6, -7
 F32, F64 = 4, 8
 PAD_CONSTANT_VF, PAD_ZERO = 3, 100
-KSPEC_REAL, KSPEC_COMPLEX_STD = 0, 1
+KSPEC_REAL, KSPEC_COMPLEX_STD, KSPEC_SEPARABLE = 0, 1, 2
 PASS_XFWD, PASS_YFWD, PASS_ZCONV, PASS_YINV, PASS_XINV = 0, 1, 2, 3, 4
 PASS_NAMES = ["xfwd", "yfwd", "zconv", "yinv", "xinv"]
 
