@@ -374,33 +374,38 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned long bytes) 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
+// Cache-policy operand of a buffer load / store: kCacheDefault, or kStream --
+// the non-temporal hint for data a pass reads exactly once or writes for a
+// later pass (it keeps streamed arrays from evicting what is reused).
+constexpr int kCacheDefault = 0, kStream = 2;
+
 template <typename T> struct BufOps;
 template <> struct BufOps<float> {
-    static __device__ __forceinline__ float2 ld_cx(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    template <int A = kCacheDefault> static __device__ __forceinline__ float2 ld_cx(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, A));
     }
-    static __device__ __forceinline__ void st_cx(float2 v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(float2 v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, A);
     }
-    static __device__ __forceinline__ float ld_r(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    template <int A = kCacheDefault> static __device__ __forceinline__ float ld_r(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, A));
     }
-    static __device__ __forceinline__ void st_r(float v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(float v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, A);
     }
 };
 template <> struct BufOps<double> {
-    static __device__ __forceinline__ double2 ld_cx(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    template <int A = kCacheDefault> static __device__ __forceinline__ double2 ld_cx(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, A));
     }
-    static __device__ __forceinline__ void st_cx(double2 v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(double2 v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, A);
     }
-    static __device__ __forceinline__ double ld_r(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    template <int A = kCacheDefault> static __device__ __forceinline__ double ld_r(rsrc_t r, unsigned off) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, A));
     }
-    static __device__ __forceinline__ void st_r(double v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(double v, rsrc_t r, unsigned off) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, A);
     }
 };
 

@@ -151,7 +151,7 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
             const cx<T> e = cscale(cadd(zk, zc), T(0.5));
             const cx<T> d = cscale(csub(zk, zc), T(0.5));
             const cx<T> wd = cmul(d, tw[k]);  // W_Px^k d
-            B::st_cx(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (unsigned)k * CB : kBufOOB);  // e - i W d
+            B::template st_cx<kStream>(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (unsigned)k * CB : kBufOOB);  // e - i W d
         }
         // the next item's first LDS store happens after a barrier inside fft_regs
     }
@@ -222,12 +222,12 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 const cx<T> val = v[b * Sh::RL + r];
                 if (vec) {
                     const bool ok = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::st_cx(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (unsigned)(2 * n) * RB : kBufOOB);
+                    B::template st_cx<kStream>(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (unsigned)(2 * n) * RB : kBufOOB);
                 } else {
                     const bool ok0 = n >= 0 && live && 2 * n < g.Nx;
                     const bool ok1 = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::st_r(val.x + addc, rout, ok0 ? orow + (unsigned)(2 * n) * RB : kBufOOB);
-                    B::st_r(val.y + addc, rout, ok1 ? orow + (unsigned)(2 * n + 1) * RB : kBufOOB);
+                    B::template st_r<kStream>(val.x + addc, rout, ok0 ? orow + (unsigned)(2 * n) * RB : kBufOOB);
+                    B::template st_r<kStream>(val.y + addc, rout, ok1 ? orow + (unsigned)(2 * n + 1) * RB : kBufOOB);
                 }
             }
         // the next item's first LDS store happens after a barrier inside fft_regs
@@ -287,7 +287,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShF::R0; ++r) {
                 const int n = ShF::in_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nin && live;
-                x[b * ShF::R0 + r] = B::ld_cx(rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
+                x[b * ShF::R0 + r] = B::template ld_cx<kStream>(rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
             }
     }
     if constexpr (MODE == COL_INV) {
@@ -413,7 +413,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 for (int r = 0; r < ShF::RL; ++r) {
                     const int m = ShF::out_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    B::st_cx(v[b * ShF::RL + r], rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
+                    B::template st_cx<kStream>(v[b * ShF::RL + r], rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
                 }
             continue;
         } else {
@@ -440,7 +440,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShI::RL; ++r) {
                 const int n = ShI::out_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nout && live;
-                B::st_cx(acc[b * ShI::RL + r], rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
+                B::template st_cx<kStream>(acc[b * ShI::RL + r], rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
             }
     }
 }

@@ -9,7 +9,8 @@ import ctypes
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libvfd.so")
+# VFD_LIBRARY selects another build of the library (same C-ABI)
+LIB_PATH = os.environ.get("VFD_LIBRARY") or os.path.join(_PKG, "libvfd.so")
 
 VFD_OK = 0
 VFD_ERR_INVALID, VFD_ERR_UNSUPPORTED, VFD_ERR_DEVICE = -1, -2, -3
