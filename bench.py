@@ -242,6 +242,22 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(shape, nfields, kspec, pass_name):
+    """HBM bytes per launch of one pass, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs of this workload (scaled by the copy-kernel calibration);
+    None when no measurement of this workload is committed."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{shape[2]}x{shape[1]}x{shape[0]}_{nfields}f.json")
+    try:
+        with open(path) as f:
+            m = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if m.get("kspec") != kspec or m.get("fields") != nfields or list(m.get("shape", [])) != list(shape):
+        return None
+    b = m.get("per_launch_bytes", {}).get(pass_name)
+    return None if b is None else float(b["read"] + b["write"])
+
+
 def synth_fields(shape, nfields, seed, device):
     import numpy as np
     import torch
@@ -381,7 +397,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": pass_gbs[dom] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(shape, nf, args.kspec, vfd.PASS_NAMES[dom]),
             "step_frac": step_bytes / (mean_step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "passes_ms": dict(zip(vfd.PASS_NAMES, pass_ms)),
             "passes_gbs": dict(zip(vfd.PASS_NAMES, pass_gbs)),
