@@ -114,6 +114,47 @@ def bench_copy(args):
     print(json.dumps(out), flush=True)
 
 
+def reference_flow_tick(picks, nf, seed, ticks=2):
+    """One tick of every field of every run the way the reference's
+    diffuseChemGPU drives the GPU: host->device, each pass followed by a
+    device synchronization, device->host, one field at a time."""
+    import numpy as np
+    import torch
+    import vfd
+    rng = np.random.default_rng(seed + 1)
+    plans, work, dev, spec = {}, {}, {}, {}
+    host = []
+    for g in sorted(set(int(x) for x in picks)):
+        shape = ENSEMBLE_GRIDS[g]
+        plans[g] = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+        work[g] = plans[g].new_work()
+        dev[g] = torch.empty(shape, dtype=torch.float32, device="cuda:0")   # one device field per shape
+        s2, dc = _run_params(rng, 1)
+        spec[g] = plans[g].gauss_spectrum(s2[0], dc[0])
+    for g in picks:
+        host.append([torch.rand(ENSEMBLE_GRIDS[g], dtype=torch.float32).pin_memory() for _ in range(nf)])
+    torch.cuda.synchronize()
+
+    def tick():
+        for i, g in enumerate(picks):
+            g = int(g)
+            ent = plans[g].field_array([{"in": dev[g], "out": dev[g], "work": work[g], "kspec": spec[g]}])
+            for f in range(nf):
+                dev[g].copy_(host[i][f])
+                torch.cuda.synchronize()
+                for p in range(5):
+                    plans[g].diffuse_pass(ent, p)
+                    torch.cuda.synchronize()
+                host[i][f].copy_(dev[g])
+                torch.cuda.synchronize()
+
+    tick()
+    a = time.perf_counter()
+    for _ in range(ticks):
+        tick()
+    return (time.perf_counter() - a) / ticks
+
+
 def bench_ensemble(args):
     """Batched (graph) vs plain per-field ticks over uneven resident runs."""
     import numpy as np
@@ -149,6 +190,7 @@ def bench_ensemble(args):
     t_graph = timed(lambda: eng.step(runs, graph=True, stream=stream), K)
     t_batch = timed(lambda: eng.step(runs, graph=False, stream=stream), K)
     t_plain = timed(lambda: eng.step_plain(runs, stream=stream), max(1, K // 2))
+    t_ref = reference_flow_tick(picks, nf, args.seed)
     counts = {f"{s[2]}x{s[1]}x{s[0]}": int((picks == i).sum()) for i, s in enumerate(ENSEMBLE_GRIDS)}
     bytes_tick = 0.0
     for i, s in enumerate(ENSEMBLE_GRIDS):
@@ -164,6 +206,11 @@ def bench_ensemble(args):
     out["batched_eager"] = {"ms_per_tick": t_batch * 1e3, "voxel_updates_s": vox / t_batch}
     out["plain_per_field"] = {"ms_per_tick": t_plain * 1e3, "voxel_updates_s": vox / t_plain}
     out["speedup_vs_plain"] = t_plain / t_graph
+    out["reference_flow"] = {"ms_per_tick": t_ref * 1e3, "voxel_updates_s": vox / t_ref,
+                             "what": "the reference's per-field drop-in flow on these kernels: pinned host field -> "
+                                     "device, 5 pass launches each followed by a device synchronization, device -> "
+                                     "host, one field at a time (Diffuser.cpp:300-421, Diffusion.cu:1802-1870)"}
+    out["speedup_vs_reference_flow"] = t_ref / t_graph
     out["algorithmic_gbs"] = bytes_tick / t_graph / 1e9
     out["setup_s"] = setup_s
     out["engine"] = eng.stats()

@@ -135,6 +135,22 @@ int vfd_field_mass(const vfd_plan* plan, const void* const* d_fields, int nfield
 int vfd_r2c_3d(const vfd_plan* plan, const void* d_real, void* d_complex, void* stream);
 int vfd_c2r_3d(const vfd_plan* plan, void* d_complex, void* d_real, void* stream);
 
+/* Post-diffusion chem update: WHChemical::update with gpuUpdate
+ * (src/FieldVariable/Usr_FieldVariables/WHChemical.cpp:395-450) on the device.
+ * For the 8 chems (enums.h:124-133: TNF, TGF, FGF, MMP8, IL1beta, IL6, IL8,
+ * IL10) and n voxels: p = d + t * decay[ic] (FP32, multiply and add rounded
+ * separately), d = 0; then the gradients (d_grads in the order NEU, MAC, FIB,
+ * enums.h:134-136):
+ *   g   = p[IL1beta] + p[TNF] + p[TGF] + p[FGF] + frag_col   (left to right)
+ *   NEU = g + p[IL6] + p[IL8];  MAC = g + frag_ela;  FIB = p[TGF].
+ * d_t: the diffused fields (tChem); d_d: the agents' deltas (dChem, cleared);
+ * d_p: the new levels (pChem; may alias d_t).  totals (8 doubles, optional)
+ * receives compensated FP64 sums of each p; a non-null totals synchronizes
+ * stream. */
+int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_p, const float* decay,
+                    const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,
+                    double* totals, void* stream);
+
 /* ======================================================================
  * Engine: resident runs, batched ticks, serving.
  *

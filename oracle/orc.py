@@ -65,6 +65,9 @@ def lib():
         L.cpu_fft_diffuse3d_f32.argtypes = [_p, _p, _i, _i, _i, _p, _i]
         L.orc_gauss_spectrum_real.argtypes = [_d, _d, _i, _i, _i, _d, _p]
         L.orc_num_threads.restype = _i
+        L.orc_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
+                                      ctypes.POINTER(ctypes.c_float), _p, _p, _p, _p, _p, ctypes.c_long,
+                                      ctypes.POINTER(_d)]
         _lib = L
     return _lib
 
@@ -276,3 +279,22 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def chem_update(t, d, decay, frag_col, frag_ela):
+    """WHChemical::update (gpuUpdate) restated: returns (p, d_after, (neu, mac, fib), totals)
+    for 8 FP32 arrays t, d of n voxels."""
+    n = t[0].size
+    t = [np.ascontiguousarray(x, dtype=np.float32).ravel() for x in t]
+    d = [np.ascontiguousarray(x, dtype=np.float32).ravel().copy() for x in d]
+    p = [np.empty(n, dtype=np.float32) for _ in range(8)]
+    fc = np.ascontiguousarray(frag_col, dtype=np.float32).ravel()
+    fe = np.ascontiguousarray(frag_ela, dtype=np.float32).ravel()
+    neu, mac, fib = (np.empty(n, dtype=np.float32) for _ in range(3))
+    P8 = ctypes.c_void_p * 8
+    tot = (_d * 8)()
+    dec = (ctypes.c_float * 8)(*[float(x) for x in decay])
+    assert lib().orc_chem_update(P8(*[a.ctypes.data for a in t]), P8(*[a.ctypes.data for a in d]),
+                                 P8(*[a.ctypes.data for a in p]), dec, _ptr(fc), _ptr(fe), _ptr(neu),
+                                 _ptr(mac), _ptr(fib), n, tot) == 0
+    return p, d, (neu, mac, fib), list(tot)

@@ -857,3 +857,38 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+
+/* ------------------------------------------------------------------------- */
+/* WHChemical::update with gpuUpdate (WHChemical.cpp:395-450): per voxel, for  */
+/* the 8 chems (enums.h:124-133) level = d + t * decay (FP32, the product      */
+/* rounded before the add), d = 0, p = level; gradients (WHChemical.cpp:       */
+/* 437-445) in the reference's left-to-right FP32 order; totals as            */
+/* compensated FP64 sums of p.  grads = NEU, MAC, FIB (enums.h:134-136).       */
+/* ------------------------------------------------------------------------- */
+int orc_chem_update(const float *const *t, float *const *d, float *const *p, const float *decay,
+                    const float *frag_col, const float *frag_ela, float *neu, float *mac, float *fib,
+                    long n, double *totals)
+{
+    enum { TNF = 0, TGF = 1, FGF = 2, IL1beta = 4, IL6 = 5, IL8 = 6 };
+    double s[8] = {0}, c[8] = {0};
+    for (long i = 0; i < n; i++) {
+        float lv[8];
+        for (int ic = 0; ic < 8; ic++) {
+            volatile float scaled = t[ic][i] * decay[ic];
+            lv[ic] = d[ic][i] + scaled;
+            d[ic][i] = 0.0f;
+            p[ic][i] = lv[ic];
+            double x = lv[ic], tt = s[ic] + x;
+            c[ic] += fabs(s[ic]) >= fabs(x) ? (s[ic] - tt) + x : (x - tt) + s[ic];
+            s[ic] = tt;
+        }
+        float grad = lv[IL1beta] + lv[TNF] + lv[TGF] + lv[FGF] + frag_col[i];
+        fib[i] = lv[TGF];
+        neu[i] = grad + lv[IL6] + lv[IL8];
+        mac[i] = grad + frag_ela[i];
+    }
+    if (totals)
+        for (int ic = 0; ic < 8; ic++) totals[ic] = s[ic] + c[ic];
+    return 0;
+}

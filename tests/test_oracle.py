@@ -267,3 +267,28 @@ def test_separable_f64_input_matches_f32_entry(orc):
     r1f = orc.direct_separable(a.astype(np.float32), 1.3, 0.97)
     assert not np.array_equal(r1, r1f)
     assert orc.rel_l2(r1f, r1) < 1e-7
+
+
+def test_chem_update_known_answers(orc):
+    """WHChemical::update (gpuUpdate): hand-computed levels, gradients, totals."""
+    n = 5
+    t = [np.full(n, k + 1.0, np.float32) for k in range(8)]
+    d = [np.full(n, 0.5, np.float32) for _ in range(8)]
+    p, d_after, (neu, mac, fib), tot = orc.chem_update(t, d, [0.5] * 8, np.ones(n), np.full(n, 2.0))
+    for k in range(8):
+        assert np.all(p[k] == 0.5 + 0.5 * (k + 1))
+        assert np.all(d_after[k] == 0.0)
+        assert tot[k] == n * (0.5 + 0.5 * (k + 1))
+    # IL1beta 3.0 + TNF 1.0 + TGF 1.5 + FGF 2.0 + frag_col 1.0 = 8.5
+    assert np.all(fib == 1.5) and np.all(neu == 8.5 + 3.5 + 4.0) and np.all(mac == 8.5 + 2.0)
+
+
+def test_chem_update_rounds_product_before_add(orc):
+    """level = d + t*decay with the product rounded to FP32 first (no fused
+    multiply-add): (1+2^-12)^2 rounds to 1+2^-11, so d = -(1+2^-11) gives 0,
+    where a fused evaluation would give 2^-24."""
+    e = np.float32(1 + 2.0 ** -12)
+    t = [np.array([e], np.float32) for _ in range(8)]
+    d = [np.array([-(1 + 2.0 ** -11)], np.float32) for _ in range(8)]
+    p, _, _, _ = orc.chem_update(t, d, [float(e)] * 8, np.zeros(1), np.zeros(1))
+    assert all(x[0] == 0.0 for x in p)

@@ -1,0 +1,169 @@
+// vfd_chem.hip -- the post-diffusion chem update on the device.
+//
+// Restates WHChemical::update with gpuUpdate (src/FieldVariable/Usr_FieldVariables/
+// WHChemical.cpp:395-450), which runs on the host after every diffusion tick:
+// per voxel and chem, p = d + t * decay[ic] (the diffused field t scaled by the
+// chem's decay, plus the agents' deltas d), d = 0, per-chem totals, then the
+// three chemotaxis gradient fields (NEU, MAC, FIB; enums.h:134-136) from the
+// new levels and the ECM fragment fields.  With the fields resident on the
+// device, this removes the host pass over 8 + 8 + 8 + 3 arrays per tick.
+//
+// Arithmetic follows the reference in FP32 with the multiply and the add
+// rounded separately (no fused multiply-add), and the gradient sums in the
+// reference's left-to-right order, so levels and gradients are bit-identical
+// to a host evaluation of the same expressions.  Totals are compensated
+// (Neumaier) FP64 sums with a fixed reduction order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/vfd.h"
+#include "vfd_internal.h"
+
+using namespace vfd;
+
+namespace {
+
+constexpr int kChems = 8;
+constexpr int kThreads = 256;
+constexpr int kBlocks = 1024;
+// chem indices (enums.h:124-133)
+constexpr int TNF = 0, TGF = 1, FGF = 2, IL1beta = 4, IL6 = 5, IL8 = 6;
+
+struct ChemPtrs {
+    const float* t[kChems];
+    float* d[kChems];
+    float* p[kChems];
+    float decay[kChems];
+};
+
+__device__ __forceinline__ void nadd(double& s, double& c, double x) {
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+// Each block owns one contiguous chunk of voxels; lanes stride through it and
+// a fixed-order tree combines the per-lane (sum, compensation) pairs.
+__global__ void __launch_bounds__(kThreads)
+chem_update_kernel(ChemPtrs cp, const float* __restrict__ frag_col, const float* __restrict__ frag_ela,
+                   float* __restrict__ neu, float* __restrict__ mac, float* __restrict__ fib, long n, long chunk,
+                   double2* __restrict__ partial) {
+#pragma clang fp contract(off)  // the product is rounded before the add, as on the host
+    const long lo = (long)blockIdx.x * chunk;
+    const long hi = lo + chunk < n ? lo + chunk : n;
+    double s[kChems], c[kChems];
+#pragma unroll
+    for (int ic = 0; ic < kChems; ++ic) s[ic] = c[ic] = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += kThreads) {
+        float lv[kChems];
+#pragma unroll
+        for (int ic = 0; ic < kChems; ++ic) {
+            const float scaled = cp.t[ic][i] * cp.decay[ic];
+            lv[ic] = cp.d[ic][i] + scaled;
+        }
+#pragma unroll
+        for (int ic = 0; ic < kChems; ++ic) {
+            cp.d[ic][i] = 0.0f;
+            cp.p[ic][i] = lv[ic];
+            nadd(s[ic], c[ic], (double)lv[ic]);
+        }
+        // gradients (WHChemical.cpp:437-445), left-to-right FP32 sums
+        const float grad = lv[IL1beta] + lv[TNF] + lv[TGF] + lv[FGF] + frag_col[i];
+        fib[i] = lv[TGF];
+        neu[i] = grad + lv[IL6] + lv[IL8];
+        mac[i] = grad + frag_ela[i];
+    }
+    __shared__ double ss[kChems][kThreads], sc[kChems][kThreads];
+#pragma unroll
+    for (int ic = 0; ic < kChems; ++ic) {
+        ss[ic][threadIdx.x] = s[ic];
+        sc[ic][threadIdx.x] = c[ic];
+    }
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int ic = 0; ic < kChems; ++ic) {
+                double a = ss[ic][threadIdx.x], ca = sc[ic][threadIdx.x];
+                const double b = ss[ic][threadIdx.x + w], cb = sc[ic][threadIdx.x + w];
+                nadd(a, ca, b);
+                ss[ic][threadIdx.x] = a;
+                sc[ic][threadIdx.x] = ca + cb;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kChems)
+        partial[(long)threadIdx.x * gridDim.x + blockIdx.x] = double2{ss[threadIdx.x][0], sc[threadIdx.x][0]};
+}
+
+// one block per chem: the block partials in a fixed order
+__global__ void __launch_bounds__(kThreads) chem_total_kernel(const double2* __restrict__ partial, int nparts,
+                                                              double* __restrict__ totals) {
+    const double2* pp = partial + (long)blockIdx.x * nparts;
+    double s = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kThreads) {
+        nadd(s, c, pp[i].x);
+        c += pp[i].y;
+    }
+    __shared__ double ss[kThreads], sc[kThreads];
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double a = ss[threadIdx.x], ca = sc[threadIdx.x];
+            nadd(a, ca, ss[threadIdx.x + w]);
+            ss[threadIdx.x] = a;
+            sc[threadIdx.x] = ca + sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = ss[0] + sc[0];
+}
+
+}  // namespace
+
+extern "C" int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_p, const float* decay,
+                               const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,
+                               double* totals, void* stream) {
+    if (!d_t || !d_d || !d_p || !decay || !d_frag_col || !d_frag_ela || !d_grads)
+        return set_error(VFD_ERR_INVALID, "chem_update: null argument");
+    if (n < 0) return set_error(VFD_ERR_INVALID, "chem_update: negative voxel count");
+    ChemPtrs cp;
+    for (int ic = 0; ic < kChems; ++ic) {
+        if (!d_t[ic] || !d_d[ic] || !d_p[ic])
+            return set_error(VFD_ERR_INVALID, "chem_update: chem %d has a null field", ic);
+        cp.t[ic] = d_t[ic];
+        cp.d[ic] = d_d[ic];
+        cp.p[ic] = d_p[ic];
+        cp.decay[ic] = decay[ic];
+    }
+    for (int g = 0; g < 3; ++g)
+        if (!d_grads[g]) return set_error(VFD_ERR_INVALID, "chem_update: gradient %d is null", g);
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (int)std::min<long>(kBlocks, std::max<long>(1, (n + kThreads - 1) / kThreads));
+    const long chunk = (n + nb - 1) / nb;
+    double2* partial = nullptr;
+    double* d_tot = nullptr;
+    const size_t pbytes = (size_t)kChems * nb * sizeof(double2);
+    if (int rc = hip_check(hipMallocAsync((void**)&partial, pbytes + kChems * sizeof(double), st),
+                           "hipMallocAsync(chem update)"))
+        return rc;
+    d_tot = (double*)((char*)partial + pbytes);
+    hipLaunchKernelGGL(chem_update_kernel, dim3(nb), dim3(kThreads), 0, st, cp, d_frag_col, d_frag_ela,
+                       d_grads[0], d_grads[1], d_grads[2], n, chunk, partial);
+    int rc = hip_check(hipGetLastError(), "chem update launch");
+    if (!rc) {
+        hipLaunchKernelGGL(chem_total_kernel, dim3(kChems), dim3(kThreads), 0, st, partial, nb, d_tot);
+        rc = hip_check(hipGetLastError(), "chem totals launch");
+    }
+    if (!rc && totals) {
+        rc = hip_check(hipMemcpyAsync(totals, d_tot, kChems * sizeof(double), hipMemcpyDeviceToHost, st),
+                       "chem totals download");
+        if (!rc) rc = hip_check(hipStreamSynchronize(st), "chem totals sync");
+    }
+    (void)hipFreeAsync(partial, st);
+    return rc;
+}

@@ -29,6 +29,9 @@ namespace {
 constexpr int kMaxRunFields = 32;
 constexpr int kAuxStreams = 8;
 constexpr int kMaxGraphs = 8;
+// work buffers one shape group of a batch may hold: enough fields per launch
+// to fill the GPU on small grids, without pooling tens of GB for large ones
+constexpr size_t kGroupWorkBytes = size_t(8) << 30;
 
 struct ShapeKey {
     int nz, ny, nx, prec;
@@ -193,7 +196,8 @@ int build_batch(vfd_engine* e, const std::vector<int>& run_ids, Batch* bt) {
     for (auto& kv : groups) {
         Shape& s = e->shapes[kv.first];
         Held h;
-        const int want = std::min((int)kv.second.size(), kMaxFieldsPerLaunch);
+        const int by_bytes = (int)std::max<size_t>(1, kGroupWorkBytes / s.plan.work_bytes);
+        const int want = std::min({(int)kv.second.size(), kMaxFieldsPerLaunch, by_bytes});
         if (acquire(e, s, want, &h) < 1) {
             release(h);
             for (auto& hh : bt->held) release(hh);

@@ -70,6 +70,9 @@ lib.vfd_plan_set_schedule.argtypes = [_p, ctypes.POINTER(_i)]
 lib.vfd_gauss_spectrum.argtypes = [_p, _d, _d, _p, _p]
 lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _p]
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
+lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
+                                ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
+                                ctypes.POINTER(_d), _p]
 lib.vfd_c2r_3d.argtypes = [_p, _p, _p, _p]
 
 
@@ -197,3 +200,23 @@ class Plan:
 
 def length_supported(q):
     return bool(lib.vfd_length_supported(int(q)))
+
+
+def chem_update(t, d, p, decay, frag_col, frag_ela, grads, stream=None, totals=True):
+    """WHChemical::update (gpuUpdate) on the device for 8 chems of n voxels:
+    p = d + t * decay, d = 0, gradients (NEU, MAC, FIB) from p and the ECM
+    fragment fields.  t, d, p: 8 FP32 device tensors each (p may be t);
+    grads: 3 FP32 device tensors.  Returns the 8 compensated totals of p (and
+    synchronizes the stream) unless totals=False."""
+    n = t[0].numel()
+    for a in list(t) + list(d) + list(p) + [frag_col, frag_ela] + list(grads):
+        if a.numel() != n or not a.is_contiguous():
+            raise ValueError("chem_update: every array must be contiguous with the same voxel count")
+    P8 = ctypes.c_void_p * 8
+    tv, dv, pv = P8(*[x.data_ptr() for x in t]), P8(*[x.data_ptr() for x in d]), P8(*[x.data_ptr() for x in p])
+    gv = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in grads])
+    dec = (ctypes.c_float * 8)(*[float(x) for x in decay])
+    out = (ctypes.c_double * 8)()
+    check(lib.vfd_chem_update(tv, dv, pv, dec, _ptr(frag_col), _ptr(frag_ela), gv, n,
+                              out if totals else None, _stream(stream)))
+    return list(out) if totals else None
