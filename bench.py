@@ -22,6 +22,11 @@ Other workloads (one JSON line each, same fields where they apply):
                        plain per-field form (one field at a time, per-pass
                        launches, no graph).
   --workload copy      1 GiB device-to-device copy: measured HBM bandwidth.
+  --workload precision 8 fields on 1536x768x192, log-uniform background in
+                       [1e-6, 1e3] plus 500 sharp sources: the FP32 path against
+                       the full-FP64 GPU path over 100 ticks (max-abs, rel-L2,
+                       mass drift per field) and both against the FP64 oracle
+                       after the first tick.
   --workload serve     Poisson request stream (mean 200/s) over resident runs
                        drawn from that set plus 1024x512x128, admitted against a
                        200 GB HBM budget; continuous batching; throughput and
@@ -54,7 +59,8 @@ def parse():
                     help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
     ap.add_argument("--kspec", default="separable", choices=["separable", "real"],
                     help="kernel-spectrum format: FP64 1D tables formed in the z pass, or the 3D real array")
-    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve", "copy"])
+    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve", "copy", "precision"])
+    ap.add_argument("--ticks", type=int, default=100, help="precision: ticks of FP32 and FP64 side by side")
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
@@ -153,6 +159,96 @@ def reference_flow_tick(picks, nf, seed, ticks=2):
     for _ in range(ticks):
         tick()
     return (time.perf_counter() - a) / ticks
+
+
+def precision_fields(shape, nfields, seed, device):
+    """Log-uniform background in [1e-6, 1e3] plus 500 sharp sources per field."""
+    import numpy as np
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    n = shape[0] * shape[1] * shape[2]
+    out = []
+    for _ in range(nfields):
+        f = torch.rand(shape, generator=g, device=device, dtype=torch.float64)
+        f = torch.pow(10.0, f * 9.0 - 6.0)
+        idx = torch.randint(0, n, (500,), generator=g, device=device)
+        amp = torch.rand(500, generator=g, device=device, dtype=torch.float64) * 500.0 + 500.0
+        f.view(-1).index_add_(0, idx, amp)
+        out.append(f.float())   # the FP32 field both paths start from
+    return out
+
+
+def bench_precision(args):
+    """FP32 vs full-FP64 GPU path over many ticks; the FP64 oracle after tick 1."""
+    import numpy as np
+    import torch
+    import vfd
+    shape = (192, 768, 1536) if args.shape == "128,512,1024" else tuple(int(s) for s in args.shape.split(","))
+    nf = args.fields
+    rng = np.random.default_rng(args.seed)
+    s2, dc = _run_params(rng, nf)
+    p32 = vfd.Plan(shape, precision=vfd.F32, kspec_format=vfd.KSPEC_SEPARABLE)
+    p64 = vfd.Plan(shape, precision=vfd.F64, kspec_format=vfd.KSPEC_SEPARABLE)
+    u0 = precision_fields(shape, nf, args.seed, "cuda:0")
+    f32 = [u.clone() for u in u0]
+    f64 = [u.double() for u in u0]
+    k32 = [p32.gauss_spectrum(a, b) for a, b in zip(s2, dc)]
+    k64 = [p64.gauss_spectrum(a, b) for a, b in zip(s2, dc)]
+    w32, w64 = p32.new_work(), p64.new_work()   # fields run one at a time: one work buffer each
+    arr32 = [p32.field_array([{"in": f32[i], "out": f32[i], "work": w32, "kspec": k32[i]}]) for i in range(nf)]
+    arr64 = [p64.field_array([{"in": f64[i], "out": f64[i], "work": w64, "kspec": k64[i]}]) for i in range(nf)]
+    report = {}
+
+    def compare(tick):
+        m32, m64 = p32.mass(f32), p64.mass(f64)
+        rows = []
+        for i in range(nf):
+            a, b = f32[i].double(), f64[i]
+            err = (a - b).abs()
+            rows.append({"field": i, "max_abs": float(err.max()), "rel_l2": float(torch.linalg.vector_norm(a - b) /
+                                                                                torch.linalg.vector_norm(b)),
+                         "mass_drift_fp32_vs_fp64": abs(m32[i] - m64[i]) / abs(m64[i])})
+        report[str(tick)] = rows
+
+    t0 = time.perf_counter()
+    oracle = {}
+    for tick in range(1, args.ticks + 1):
+        for i in range(nf):
+            p32.diffuse(arr32[i])
+            p64.diffuse(arr64[i])
+        if tick == 1:
+            torch.cuda.synchronize()
+            # FP64 oracle (separable direct convolution) on field 0 after one tick
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import orc
+            ref = orc.direct_separable(u0[0].cpu().numpy(), s2[0], dc[0])
+            oracle = {"field": 0, "rel_l2_fp64_gpu": orc.rel_l2(f64[0].cpu().numpy(), ref),
+                      "rel_l2_fp32_gpu": orc.rel_l2(f32[0].cpu().numpy(), ref)}
+        if tick in (1, 10, args.ticks):
+            torch.cuda.synchronize()
+            compare(tick)
+        print(f"precision: tick {tick} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    last = report[str(args.ticks)]
+    worst = max(r["rel_l2"] for r in last)
+    out = _header(args, f"precision stress: worst per-field rel-L2 of FP32 vs the FP64 GPU path after {args.ticks} ticks",
+                  worst, "rel-L2", wall * 1e3 / args.ticks,
+                  {"workload": f"{nf} fields, {shape[2]}x{shape[1]}x{shape[0]} grid, log-uniform background in "
+                               "[1e-6, 1e3] plus 500 sharp sources of amplitude U[500, 1000], zero-padded linear "
+                               "convolution, Gaussian kernels as the tissue-scale workload",
+                   "ticks": args.ticks}, dtype="f32 vs f64")
+    out["higher_is_better"] = False
+    out["per_tick"] = report
+    out["oracle_tick1"] = oracle
+    out["max_rel_l2"] = worst
+    out["max_mass_drift"] = max(r["mass_drift_fp32_vs_fp64"] for r in last)
+    # FP32 twiddle rounding gives every frequency a small systematic gain, so
+    # the drift against FP64 grows linearly with ticks: report it per tick too
+    out["max_mass_drift_per_tick"] = out["max_mass_drift"] / args.ticks
+    out["max_rel_l2_per_tick"] = worst / args.ticks
+    print(json.dumps(out), flush=True)
 
 
 def bench_ensemble(args):
@@ -366,6 +462,8 @@ def main():
         return bench_serve(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     shape = tuple(int(s) for s in args.shape.split(","))
     nz, ny, nx = shape
     dev = "cuda:0"

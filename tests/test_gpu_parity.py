@@ -140,3 +140,74 @@ def test_separable_spectrum_bit_identical_to_real(vfd, orc, shape, precision):
         a, b = ent_r[f]["out"].cpu().numpy(), ent_s[f]["out"].cpu().numpy()
         assert np.array_equal(a, b), f"field {f}"
         assert _report(f"field {f}", b, refs[f]) <= (REL_F32 if precision == 4 else REL_F64)
+
+
+def test_f64_arrays_beyond_4gib(vfd, orc):
+    """The FP64 path addresses its arrays with 64-bit offsets: at 1536x768x192
+    the FP64 work array is 7.4 GB.  One tick there agrees with the FP32 path
+    (whose arrays stay below 4 GiB) to FP32 accuracy."""
+    import torch
+    shape = (192, 768, 1536)
+    print(f"case f64_beyond_4gib shape={shape}", flush=True)
+    p64 = vfd.Plan(shape, precision=vfd.F64, kspec_format=vfd.KSPEC_SEPARABLE)
+    p32 = vfd.Plan(shape, precision=vfd.F32, kspec_format=vfd.KSPEC_SEPARABLE)
+    assert p64.info.work_bytes > (1 << 32)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    u = torch.rand(shape, generator=g, device="cuda")
+    u.view(-1)[::9973] += 100.0
+    u64 = u.double()
+    o32, o64 = torch.empty_like(u), torch.empty_like(u64)
+    p32.diffuse([{"in": u, "out": o32, "work": p32.new_work(), "kspec": p32.gauss_spectrum(3.1, 0.97)}])
+    p64.diffuse([{"in": u64, "out": o64, "work": p64.new_work(), "kspec": p64.gauss_spectrum(3.1, 0.97)}])
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.vector_norm(o32.double() - o64) / torch.linalg.vector_norm(o64))
+    m32, m64 = p32.mass([o32])[0], p64.mass([o64])[0]
+    print(f"rel-L2 FP32 vs FP64 {rel:.3e}, mass drift {abs(m32 - m64) / m64:.3e}", flush=True)
+    assert rel <= REL_F32
+    assert abs(m32 - m64) / m64 <= MASS_F32
+    # the last plane of the last rows (the top of the 7.4 GB array) is filled
+    assert torch.isfinite(o64[-1]).all() and float(o64[-1].abs().sum()) > 0
+
+
+def test_precision_many_ticks(vfd, orc):
+    """100 ticks on a wide-dynamic-range field (log-uniform 1e-6..1e3 plus
+    sharp sources): FP64 GPU against the chained FP64 oracle, FP32 GPU against
+    FP64 GPU; max-abs, rel-L2 and mass drift reported per checkpoint."""
+    import torch
+    shape = (24, 40, 48)
+    print(f"case precision_many_ticks shape={shape}", flush=True)
+    rng = np.random.default_rng(11)
+    u = (10.0 ** rng.uniform(-6, 3, shape)).astype(np.float32)
+    u.reshape(-1)[rng.integers(0, u.size, 20)] += np.float32(800.0)
+    s2, dec = 2.0 * 1.7, float(np.exp(-0.01))
+    p64 = vfd.Plan(shape, precision=vfd.F64, kspec_format=vfd.KSPEC_SEPARABLE)
+    p32 = vfd.Plan(shape, precision=vfd.F32, kspec_format=vfd.KSPEC_SEPARABLE)
+    f64 = torch.from_numpy(u.astype(np.float64)).cuda()
+    f32 = torch.from_numpy(u).cuda()
+    # field_array keeps raw pointers: the work and spectrum tensors must stay referenced
+    w64, k64 = p64.new_work(), p64.gauss_spectrum(s2, dec)
+    w32, k32 = p32.new_work(), p32.gauss_spectrum(s2, dec)
+    a64 = p64.field_array([{"in": f64, "out": f64, "work": w64, "kspec": k64}])
+    a32 = p32.field_array([{"in": f32, "out": f32, "work": w32, "kspec": k32}])
+    ref = u.astype(np.float64)
+    for tick in range(1, 101):
+        p64.diffuse(a64)
+        p32.diffuse(a32)
+        ref = orc.direct_separable(ref, s2, dec)
+        if tick in (1, 10, 100):
+            torch.cuda.synchronize()
+            g64, g32 = f64.cpu().numpy(), f32.cpu().numpy().astype(np.float64)
+            r64 = orc.rel_l2(g64, ref)
+            r32 = orc.rel_l2(g32, g64)
+            m64, m32 = p64.mass([f64])[0], p32.mass([f32])[0]
+            mref = orc.neumaier_sum(ref)
+            print(f"tick {tick}: FP64 vs oracle rel-L2 {r64:.3e} max-abs {np.abs(g64 - ref).max():.3e} "
+                  f"mass drift {abs(m64 - mref) / mref:.3e}; FP32 vs FP64 rel-L2 {r32:.3e} "
+                  f"max-abs {np.abs(g32 - g64).max():.3e} mass drift {abs(m32 - m64) / m64:.3e}", flush=True)
+            assert r64 <= 1e-12 * tick
+            assert abs(m64 - mref) / mref <= 1e-12
+            # per-tick bounds: FP32 twiddle rounding gives each frequency a
+            # systematic gain error, so drift accumulates linearly in ticks
+            assert r32 <= REL_F32 * tick
+            assert abs(m32 - m64) / m64 <= MASS_F32 * tick

@@ -380,32 +380,58 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kCacheDefault = 0, kStream = 2;
 
 template <typename T> struct BufOps;
+// FP32: raw buffer instructions on a per-array resource with 32-bit offsets
+// (arrays below kBufLimit); a masked lane passes OOB and reads 0 / drops its store.
 template <> struct BufOps<float> {
-    template <int A = kCacheDefault> static __device__ __forceinline__ float2 ld_cx(rsrc_t r, unsigned off) {
+    using H = rsrc_t;
+    using O = unsigned;
+    static constexpr O OOB = kBufOOB;
+    static __device__ __forceinline__ H make(const void* p, unsigned long bytes) { return make_rsrc(p, bytes); }
+    template <int A = kCacheDefault> static __device__ __forceinline__ float2 ld_cx(H r, O off) {
         return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, A));
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(float2 v, rsrc_t r, unsigned off) {
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(float2 v, H r, O off) {
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, A);
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ float ld_r(rsrc_t r, unsigned off) {
+    template <int A = kCacheDefault> static __device__ __forceinline__ float ld_r(H r, O off) {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, A));
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(float v, rsrc_t r, unsigned off) {
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(float v, H r, O off) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, A);
     }
 };
+// FP64: global addresses with 64-bit byte offsets, so one array may exceed
+// 4 GiB (the FP64 work array of a 1536x768x192 grid is 7.4 GB); a masked lane
+// passes OOB and its access is skipped (a load yields 0).
 template <> struct BufOps<double> {
-    template <int A = kCacheDefault> static __device__ __forceinline__ double2 ld_cx(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, A));
+    using H = char*;
+    using O = unsigned long;
+    static constexpr O OOB = ~0ul;
+    static __device__ __forceinline__ H make(const void* p, unsigned long) { return (char*)const_cast<void*>(p); }
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    template <int A = kCacheDefault> static __device__ __forceinline__ double2 ld_cx(H h, O off) {
+        if (off == OOB) return double2{0.0, 0.0};
+        const f64x2* q = (const f64x2*)(h + off);
+        if constexpr (A == kStream) return __builtin_bit_cast(double2, __builtin_nontemporal_load(q));
+        return __builtin_bit_cast(double2, *q);
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(double2 v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, A);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(double2 v, H h, O off) {
+        if (off == OOB) return;
+        f64x2* q = (f64x2*)(h + off);
+        if constexpr (A == kStream) __builtin_nontemporal_store(__builtin_bit_cast(f64x2, v), q);
+        else *q = __builtin_bit_cast(f64x2, v);
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ double ld_r(rsrc_t r, unsigned off) {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, A));
+    template <int A = kCacheDefault> static __device__ __forceinline__ double ld_r(H h, O off) {
+        if (off == OOB) return 0.0;
+        const double* q = (const double*)(h + off);
+        if constexpr (A == kStream) return __builtin_nontemporal_load(q);
+        return *q;
     }
-    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(double v, rsrc_t r, unsigned off) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, A);
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(double v, H h, O off) {
+        if (off == OOB) return;
+        double* q = (double*)(h + off);
+        if constexpr (A == kStream) __builtin_nontemporal_store(v, q);
+        else *q = v;
     }
 };
 

@@ -82,6 +82,7 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
     using C = RowCfg<T, L, V>;
     using Sh = FftShape<T, L, C::TPF, false, C::MAXR>;
     using B = BufOps<T>;
+    using O = typename B::O;
     __shared__ cx<T> smem[C::G * C::PITCH];
     __shared__ cx<T> twl[C::TW_N];
     const cx<T>* tw = twg;
@@ -101,12 +102,12 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         const int f = (int)(item / ngroups);
         const long row = (item % ngroups) * C::G + gr;
         const FieldDesc& fd = fb.f[f];
-        const rsrc_t rin = make_rsrc(fd.in, g.in_bytes);
-        const rsrc_t rwork = make_rsrc(fd.work, g.work_bytes);
+        const auto rin = B::make(fd.in, g.in_bytes);
+        const auto rwork = B::make(fd.work, g.work_bytes);
         const bool live = row < nrows;
         const int z = live ? (int)(row / g.Ny) : 0;
         const int y = live ? (int)(row % g.Ny) : 0;
-        const unsigned rowoff = (unsigned)(row * g.Nx) * RB;
+        const O rowoff = (O)(row * g.Nx) * RB;
         // constant-pad mode: interior 0<z<D-1, 0<y<H-1, 0<x<epi keeps u-b, the rest 0
         const bool row_in = live && (!cvf || (z > 0 && z < g.Nz - 1 && y > 0 && y < g.Ny - 1));
         const int xhi = cvf ? (fd.epi < g.Nx ? fd.epi : g.Nx) : g.Nx;
@@ -124,12 +125,12 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 cx<T> val;
                 if (vec) {
                     const bool ok = n >= 0 && row_in && x1 < g.Nx;
-                    val = B::ld_cx(rin, ok ? rowoff + (unsigned)x0 * RB : kBufOOB);
+                    val = B::ld_cx(rin, ok ? rowoff + (O)x0 * RB : B::OOB);
                 } else {
                     const bool ok0 = n >= 0 && row_in && x0 >= xlo && x0 < xhi;
                     const bool ok1 = n >= 0 && row_in && x1 >= xlo && x1 < xhi;
-                    const T a0 = B::ld_r(rin, ok0 ? rowoff + (unsigned)x0 * RB : kBufOOB);
-                    const T a1 = B::ld_r(rin, ok1 ? rowoff + (unsigned)x1 * RB : kBufOOB);
+                    const T a0 = B::ld_r(rin, ok0 ? rowoff + (O)x0 * RB : B::OOB);
+                    const T a1 = B::ld_r(rin, ok1 ? rowoff + (O)x1 * RB : B::OOB);
                     val = cx<T>{ok0 ? a0 - base : T(0), ok1 ? a1 - base : T(0)};
                 }
                 v[b * Sh::R0 + r] = val;
@@ -144,14 +145,14 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 if (k >= 0) lds.at(k) = v[b * Sh::RL + r];
             }
         __syncthreads();
-        const unsigned dst = (unsigned)((long)z * g.plane + (long)y * g.ld) * CB;
+        const O dst = (O)((long)z * g.plane + (long)y * g.ld) * CB;
         for (int k = t; k <= L; k += C::TPF) {
             const cx<T> zk = lds.at(k == L ? 0 : k);
             const cx<T> zc = cconj(lds.at(k == 0 ? 0 : L - k));
             const cx<T> e = cscale(cadd(zk, zc), T(0.5));
             const cx<T> d = cscale(csub(zk, zc), T(0.5));
             const cx<T> wd = cmul(d, tw[k]);  // W_Px^k d
-            B::template st_cx<kStream>(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (unsigned)k * CB : kBufOOB);  // e - i W d
+            B::template st_cx<kStream>(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (O)k * CB : B::OOB);  // e - i W d
         }
         // the next item's first LDS store happens after a barrier inside fft_regs
     }
@@ -167,6 +168,7 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
     using C = RowCfg<T, L, V>;
     using Sh = FftShape<T, L, C::TPF, false, C::MAXR>;
     using B = BufOps<T>;
+    using O = typename B::O;
     __shared__ cx<T> smem[C::G * C::PITCH];
     __shared__ cx<T> twl[C::TW_N];
     const cx<T>* tw = twg;
@@ -185,12 +187,12 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         const int f = (int)(item / ngroups);
         const long row = (item % ngroups) * C::G + gr;
         const FieldDesc& fd = fb.f[f];
-        const rsrc_t rwork = make_rsrc(fd.work, g.work_bytes);
-        const rsrc_t rout = make_rsrc(fd.out, g.out_bytes);
+        const auto rwork = B::make(fd.work, g.work_bytes);
+        const auto rout = B::make(fd.out, g.out_bytes);
         const bool live = row < nrows;
         const int z = live ? (int)(row / g.Ny) : 0;
         const int y = live ? (int)(row % g.Ny) : 0;
-        const unsigned srow = (unsigned)((long)z * g.plane + (long)y * g.ld) * CB;
+        const O srow = (O)((long)z * g.plane + (long)y * g.ld) * CB;
 
         cx<T> v[Sh::E];
 #pragma unroll
@@ -199,8 +201,8 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
             for (int r = 0; r < Sh::R0; ++r) {
                 const int k = Sh::in_elem(t, b, r);
                 const bool ok = k >= 0 && live;
-                const cx<T> ak = B::ld_cx(rwork, ok ? srow + (unsigned)k * CB : kBufOOB);
-                const cx<T> ac = cconj(B::ld_cx(rwork, ok ? srow + (unsigned)(L - k) * CB : kBufOOB));
+                const cx<T> ak = B::ld_cx(rwork, ok ? srow + (O)k * CB : B::OOB);
+                const cx<T> ac = cconj(B::ld_cx(rwork, ok ? srow + (O)(L - k) * CB : B::OOB));
                 const cx<T> xe = cadd(ak, ac);
                 const cx<T> xo = cmulc(csub(ak, ac), tw[ok ? k : 0]);  // (A - A*) W^-k
                 v[b * Sh::R0 + r] = cx<T>{xe.x - xo.y, xe.y + xo.x};    // Xe + i Xo
@@ -208,11 +210,11 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         fft_regs<T, L, C::TPF, true, false, C::MAXR>(t, v, lds, tw, 2);
         T addc = T(0);
         if (g.pad_mode == PAD_CONSTANT_VF) {
-            const rsrc_t rk = make_rsrc(fd.kspec, g.kspec_bytes);
+            const auto rk = B::make(fd.kspec, g.kspec_bytes);
             const T dc = B::ld_r(rk, 0);  // Re K^(0,0,0) (real part first in both layouts)
             addc = (T)fd.base * dc;
         }
-        const unsigned orow = (unsigned)(row * g.Nx) * RB;
+        const O orow = (O)(row * g.Nx) * RB;
         const bool vec = (g.Nx % 2 == 0);
 #pragma unroll
         for (int b = 0; b < Sh::NBL; ++b)
@@ -222,12 +224,12 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 const cx<T> val = v[b * Sh::RL + r];
                 if (vec) {
                     const bool ok = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::template st_cx<kStream>(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (unsigned)(2 * n) * RB : kBufOOB);
+                    B::template st_cx<kStream>(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (O)(2 * n) * RB : B::OOB);
                 } else {
                     const bool ok0 = n >= 0 && live && 2 * n < g.Nx;
                     const bool ok1 = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::template st_r<kStream>(val.x + addc, rout, ok0 ? orow + (unsigned)(2 * n) * RB : kBufOOB);
-                    B::template st_r<kStream>(val.y + addc, rout, ok1 ? orow + (unsigned)(2 * n + 1) * RB : kBufOOB);
+                    B::template st_r<kStream>(val.x + addc, rout, ok0 ? orow + (O)(2 * n) * RB : B::OOB);
+                    B::template st_r<kStream>(val.y + addc, rout, ok1 ? orow + (O)(2 * n + 1) * RB : B::OOB);
                 }
             }
         // the next item's first LDS store happens after a barrier inside fft_regs
@@ -256,6 +258,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     using ShF = FftShape<T, Q, C::TPF, false, C::MAXR>;                // forward / inverse-only plan
     using ShI = FftShape<T, Q, C::TPF, MODE == COL_FUSED, C::MAXR>;    // inverse plan
     using B = BufOps<T>;
+    using O = typename B::O;
     constexpr int P = Q * RS;
     constexpr unsigned CB = sizeof(cx<T>), RB = sizeof(T);
     __shared__ cx<T> smem[C::TC * Q];
@@ -263,13 +266,13 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     constexpr bool SEP = MODE == COL_FUSED && KS == KS_SEP;
     __shared__ double gzs[SEP ? P : 1];  // separable: the gz table of the z frequencies
     const FieldDesc& fd = fb.f[blockIdx.z];
-    const rsrc_t rwork = make_rsrc(fd.work, cd.work_bytes);
+    const auto rwork = B::make(fd.work, cd.work_bytes);
     const int c = threadIdx.x % C::TC;
     const int t = threadIdx.x / C::TC;
     const long col = (long)blockIdx.x * C::TC + c;
     const bool live = col < cd.ncols;
-    const unsigned base = (unsigned)((long)blockIdx.y * cd.pstride + col) * CB;
-    const unsigned rstep = (unsigned)cd.rstride * CB;
+    const O base = (O)((long)blockIdx.y * cd.pstride + col) * CB;
+    const O rstep = (O)cd.rstride * CB;
     ColLds<T, C::TC> lds{smem + c};
     constexpr int NI = ShF::NB0 * ShF::R0;
     constexpr int NO = ShI::NBL * ShI::RL;
@@ -287,7 +290,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShF::R0; ++r) {
                 const int n = ShF::in_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nin && live;
-                x[b * ShF::R0 + r] = B::template ld_cx<kStream>(rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
+                x[b * ShF::R0 + r] = B::template ld_cx<kStream>(rwork, ok ? base + (O)n * rstep : B::OOB);
             }
     }
     if constexpr (MODE == COL_INV) {
@@ -300,7 +303,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 for (int r = 0; r < ShI::R0; ++r) {
                     const int m = ShI::in_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    x[s * NII + b * ShI::R0 + r] = B::ld_cx(rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
+                    x[s * NII + b * ShI::R0 + r] = B::ld_cx(rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
                 }
     }
     if constexpr (MODE != COL_FWD) {
@@ -319,38 +322,39 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
         // separable K^(kz) = (((gx*gy)*scale)*gz[kz]) rounded to T: the numbers
         // the 3D real layout stores (gauss_product_kernel); gz staged in LDS
         using BD = BufOps<double>;
-        const rsrc_t rk = make_rsrc(fd.kspec, cd.kspec_bytes);
-        const double sc = BD::ld_r(rk, 0u);
-        const double gyv = BD::ld_r(rk, (unsigned)(cd.sep_gy + blockIdx.y) * 8u);
-        const double gxv = BD::ld_r(rk, live ? (unsigned)(1 + col) * 8u : kBufOOB);
+        using OD = typename BD::O;
+        const auto tk = BD::make(fd.kspec, cd.kspec_bytes);
+        const double sc = BD::ld_r(tk, OD(0));
+        const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + blockIdx.y) * 8u);
+        const double gxv = BD::ld_r(tk, live ? (OD)(1 + col) * 8u : BD::OOB);
         ka = (gxv * gyv) * sc;
-        for (int i = threadIdx.x; i < P; i += C::NT) gzs[i] = BD::ld_r(rk, (unsigned)(cd.sep_gz + i) * 8u);
+        for (int i = threadIdx.x; i < P; i += C::NT) gzs[i] = BD::ld_r(tk, (OD)(cd.sep_gz + i) * 8u);
     } else if constexpr (MODE == COL_FUSED) {
-        const rsrc_t rk = make_rsrc(fd.kspec, cd.kspec_bytes);
+        const auto rk = B::make(fd.kspec, cd.kspec_bytes);
         if constexpr (KS == KS_COMPLEX) {
-            const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col) * CB;
-            const unsigned kstep = (unsigned)cd.ks_row * CB;
+            const O kbase = (O)((long)blockIdx.y * cd.ks_plane + col) * CB;
+            const O kstep = (O)cd.ks_row * CB;
 #pragma unroll
             for (int b = 0; b < ShF::NBL; ++b)
 #pragma unroll
                 for (int r = 0; r < ShF::RL; ++r) {
                     const int m = ShF::out_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    kv[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (unsigned)(m * RS) * kstep : kBufOOB);
+                    kv[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (O)(m * RS) * kstep : B::OOB);
                     if constexpr (RS > 1)
-                        kvc[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (unsigned)(m * RS + 1) * kstep : kBufOOB);
+                        kvc[b * ShF::RL + r] = B::ld_cx(rk, ok ? kbase + (O)(m * RS + 1) * kstep : B::OOB);
                 }
         } else {
             static_assert(KS == KS_REAL, "streamed spectrum format");
-            const unsigned kbase = (unsigned)((long)blockIdx.y * cd.ks_plane + col * RS) * RB;
-            const unsigned kstep = (unsigned)cd.ks_row * RB;
+            const O kbase = (O)((long)blockIdx.y * cd.ks_plane + col * RS) * RB;
+            const O kstep = (O)cd.ks_row * RB;
 #pragma unroll
             for (int b = 0; b < ShF::NBL; ++b)
 #pragma unroll
                 for (int r = 0; r < ShF::RL; ++r) {
                     const int m = ShF::out_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    const unsigned off = ok ? kbase + (unsigned)m * kstep : kBufOOB;
+                    const O off = ok ? kbase + (O)m * kstep : B::OOB;
                     if constexpr (RS > 1) kv[b * ShF::RL + r] = B::ld_cx(rk, off);  // (K^[2m], K^[2m+1])
                     else kv[b * ShF::RL + r] = cx<T>{B::ld_r(rk, off), T(0)};
                 }
@@ -413,7 +417,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 for (int r = 0; r < ShF::RL; ++r) {
                     const int m = ShF::out_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    B::template st_cx<kStream>(v[b * ShF::RL + r], rwork, ok ? base + (unsigned)(m * RS + s) * rstep : kBufOOB);
+                    B::template st_cx<kStream>(v[b * ShF::RL + r], rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
                 }
             continue;
         } else {
@@ -440,7 +444,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShI::RL; ++r) {
                 const int n = ShI::out_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nout && live;
-                B::template st_cx<kStream>(acc[b * ShI::RL + r], rwork, ok ? base + (unsigned)n * rstep : kBufOOB);
+                B::template st_cx<kStream>(acc[b * ShI::RL + r], rwork, ok ? base + (O)n * rstep : B::OOB);
             }
     }
 }

@@ -195,9 +195,12 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
         q.kspec_bytes = (size_t)q.pz * q.py * q.hx * cxb;
     }
     q.field_bytes = (size_t)q.nx * q.ny * q.nz * q.prec;
-    if (q.work_bytes >= kBufLimit || q.kspec_bytes >= kBufLimit || q.field_bytes >= kBufLimit)
+    // FP32 passes address each array with 32-bit buffer offsets; FP64 passes
+    // use 64-bit addresses
+    if (q.prec == VFD_F32 &&
+        (q.work_bytes >= kBufLimit || q.kspec_bytes >= kBufLimit || q.field_bytes >= kBufLimit))
         return set_error(VFD_ERR_UNSUPPORTED,
-                         "plan: a per-field array exceeds 4 GiB (work %zu, spectrum %zu, field %zu bytes)",
+                         "plan: a per-field FP32 array exceeds 4 GiB (work %zu, spectrum %zu, field %zu bytes)",
                          q.work_bytes, q.kspec_bytes, q.field_bytes);
     // pruned 5-pass algorithmic bytes: 8N + 32 A1 + 32 A2 + 4 A3 (FP32); the
     // complex terms scale with the precision and K^ with its element size.

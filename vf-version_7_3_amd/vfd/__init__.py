@@ -149,11 +149,13 @@ class Plan:
     @staticmethod
     def field_array(fields):
         """fields: list of dicts with keys in, out, work, kspec (torch tensors),
-        optional base, epi -> ctypes array of vfd_field."""
+        optional base, epi -> ctypes array of vfd_field.  The array keeps the
+        tensors referenced, so their memory lives as long as the array does."""
         arr = (Field * len(fields))()
         for i, f in enumerate(fields):
             arr[i] = Field(_ptr(f["in"]), _ptr(f["out"]), _ptr(f["work"]), _ptr(f["kspec"]),
                            float(f.get("base", 0.0)), int(f.get("epi", 0)), 0)
+        arr._tensors = [f[k] for f in fields for k in ("in", "out", "work", "kspec")]
         return arr
 
     def diffuse(self, fields, stream=None):
