@@ -151,6 +151,14 @@ int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_
                     const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,
                     double* totals, void* stream);
 
+/* Visualisation downsample: sampleChem (src/Visualization/3D/volumeRender_kernel.cu:
+ * 215-322) into a dense array instead of a surface.  d_dst[dz][dy][dx] =
+ * min(d_src[stride*dz][stride*dy][stride*dx] * scale[chem_index], 1) for
+ * dz < D/stride, dy < H/stride, dx < W/stride, with the reference's per-chem
+ * display scales {5e4, 1e4, 1e6, 100, 1e4, 1e5, 1e5, 1e4}.  Asynchronous. */
+int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem_index, int stride, float* d_dst,
+                    void* stream);
+
 /* ======================================================================
  * Engine: resident runs, batched ticks, serving.
  *

@@ -65,6 +65,7 @@ def lib():
         L.cpu_fft_diffuse3d_f32.argtypes = [_p, _p, _i, _i, _i, _p, _i]
         L.orc_gauss_spectrum_real.argtypes = [_d, _d, _i, _i, _i, _d, _p]
         L.orc_num_threads.restype = _i
+        L.orc_sample_chem.argtypes = [_p, _i, _i, _i, _i, _i, _p]
         L.orc_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
                                       ctypes.POINTER(ctypes.c_float), _p, _p, _p, _p, _p, ctypes.c_long,
                                       ctypes.POINTER(_d)]
@@ -298,3 +299,11 @@ def chem_update(t, d, decay, frag_col, frag_ela):
                                  P8(*[a.ctypes.data for a in p]), dec, _ptr(fc), _ptr(fe), _ptr(neu),
                                  _ptr(mac), _ptr(fib), n, tot) == 0
     return p, d, (neu, mac, fib), list(tot)
+
+
+def sample_chem(u, chem, stride=3):
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    D, H, W = u.shape
+    out = np.empty((D // stride, H // stride, W // stride), dtype=np.float32)
+    assert lib().orc_sample_chem(_ptr(u), D, H, W, int(chem), int(stride), _ptr(out)) == 0
+    return out

@@ -892,3 +892,22 @@ int orc_chem_update(const float *const *t, float *const *d, float *const *p, con
         for (int ic = 0; ic < 8; ic++) totals[ic] = s[ic] + c[ic];
     return 0;
 }
+
+
+/* sampleChem_kernel (Visualization/3D/volumeRender_kernel.cu:215-300): every   */
+/* stride-th voxel times the chem's display scale, clamped above at 1.          */
+int orc_sample_chem(const float *src, int D, int H, int W, int chem, int stride, float *dst)
+{
+    static const float scale[8] = {50000.0f, 10000.0f, 1000000.0f, 100.0f,
+                                   10000.0f, 100000.0f, 100000.0f, 10000.0f};
+    if (chem < 0 || chem > 7 || stride < 1) return -1;
+    int sD = D / stride, sH = H / stride, sW = W / stride;
+    for (int z = 0; z < sD; z++)
+        for (int y = 0; y < sH; y++)
+            for (int x = 0; x < sW; x++) {
+                float v = src[((long)z * stride * H + (long)y * stride) * W + (long)x * stride] * scale[chem];
+                if (v > 1.0f) v = 1.0f;
+                dst[((long)z * sH + y) * sW + x] = v;
+            }
+    return 0;
+}

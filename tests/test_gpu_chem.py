@@ -49,3 +49,15 @@ def test_chem_update_in_place(vfd, orc):
     torch.cuda.synchronize()
     for k in range(8):
         assert np.array_equal(td[k].cpu().numpy(), p_ref[k])
+
+
+@pytest.mark.parametrize("shape,stride", [((32, 48, 64), 3), ((20, 17, 33), 2), ((64, 64, 64), 8)])
+def test_sample_chem_matches_oracle(vfd, orc, shape, stride):
+    import torch
+    print(f"case sample_chem shape={shape} stride={stride}", flush=True)
+    rng = np.random.default_rng(stride)
+    u = (rng.lognormal(-12, 3, shape) * np.where(rng.random(shape) < 0.1, -1, 1)).astype(np.float32)
+    ud = torch.from_numpy(u).cuda()
+    for chem in range(8):
+        got = vfd.sample_chem(ud, chem, stride).cpu().numpy()
+        assert np.array_equal(got, orc.sample_chem(u, chem, stride)), chem

@@ -292,3 +292,21 @@ def test_chem_update_rounds_product_before_add(orc):
     d = [np.array([-(1 + 2.0 ** -11)], np.float32) for _ in range(8)]
     p, _, _, _ = orc.chem_update(t, d, [float(e)] * 8, np.zeros(1), np.zeros(1))
     assert all(x[0] == 0.0 for x in p)
+
+
+def test_sample_chem_known_answers(orc):
+    """sampleChem (volumeRender_kernel.cu:215-300): stride-3 picks, per-chem
+    display scale, clamp above at 1 (negative values pass through)."""
+    u = np.zeros((7, 8, 10), np.float32)
+    u[3, 6, 9] = 1e-5          # picked (3, 6, 9 are multiples of 3)
+    u[3, 3, 0] = -2e-5         # picked, negative: scaled, not clamped
+    u[1, 1, 1] = 5.0           # not a multiple of 3: never sampled
+    u[0, 0, 3] = 1.0           # picked, clamped to 1
+    s = orc.sample_chem(u, 0, 3)
+    assert s.shape == (2, 2, 3)
+    assert s[0, 0, 1] == 1.0
+    assert s[1, 1, 0] == np.float32(-2e-5) * np.float32(50000.0)
+    # (3, 6, 9) -> (1, 2, 3) lies outside D/3 x H/3 x W/3 = 2 x 2 x 3 (the reference keeps dx < W/3)
+    assert np.count_nonzero(s) == 2
+    s2 = orc.sample_chem(u, 2, 3)
+    assert s2[0, 0, 1] == 1.0 and s2[1, 1, 0] == np.float32(-2e-5) * np.float32(1e6)

@@ -123,7 +123,39 @@ __global__ void __launch_bounds__(kThreads) chem_total_kernel(const double2* __r
     if (threadIdx.x == 0) totals[blockIdx.x] = ss[0] + sc[0];
 }
 
+// sampleChem_kernel (src/Visualization/3D/volumeRender_kernel.cu:215-300):
+// every stride-th voxel along each axis, times the chem's display scale,
+// clamped above at 1, into a dense [D/stride][H/stride][W/stride] array.
+
+__global__ void sample_chem_kernel(const float* __restrict__ src, int H, int W, int sD, int sH, int sW, int stride,
+                                   float scale, float* __restrict__ dst) {
+    const long total = (long)sD * sH * sW;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const long dx = i % sW, r = i / sW;
+        const long dy = r % sH, dz = r / sH;
+        float v = src[(dz * stride * H + dy * stride) * (long)W + dx * stride] * scale;
+        if (v > 1.0f) v = 1.0f;
+        dst[i] = v;
+    }
+}
+
 }  // namespace
+
+extern "C" int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem_index, int stride, float* d_dst,
+                               void* stream) {
+    if (!d_src || !d_dst) return set_error(VFD_ERR_INVALID, "sample_chem: null buffer");
+    if (D < 1 || H < 1 || W < 1 || stride < 1) return set_error(VFD_ERR_INVALID, "sample_chem: bad dimensions");
+    if (chem_index < 0 || chem_index >= kChems) return set_error(VFD_ERR_INVALID, "sample_chem: chem index %d", chem_index);
+    const int sD = D / stride, sH = H / stride, sW = W / stride;
+    const long total = (long)sD * sH * sW;
+    if (total == 0) return VFD_OK;
+    // display scales per chem (volumeRender_kernel.cu:258-294)
+    static const float scale[kChems] = {50000.0f, 10000.0f, 1000000.0f, 100.0f, 10000.0f, 100000.0f, 100000.0f, 10000.0f};
+    const unsigned blocks = (unsigned)std::min<long>(4096, (total + 255) / 256);
+    hipLaunchKernelGGL(sample_chem_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d_src, H, W, sD, sH, sW,
+                       stride, scale[chem_index], d_dst);
+    return hip_check(hipGetLastError(), "sample_chem launch");
+}
 
 extern "C" int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_p, const float* decay,
                                const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,

@@ -70,6 +70,7 @@ lib.vfd_plan_set_schedule.argtypes = [_p, ctypes.POINTER(_i)]
 lib.vfd_gauss_spectrum.argtypes = [_p, _d, _d, _p, _p]
 lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _p]
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
+lib.vfd_sample_chem.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
 lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
                                 ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
                                 ctypes.POINTER(_d), _p]
@@ -222,3 +223,14 @@ def chem_update(t, d, p, decay, frag_col, frag_ela, grads, stream=None, totals=T
     check(lib.vfd_chem_update(tv, dv, pv, dec, _ptr(frag_col), _ptr(frag_ela), gv, n,
                               out if totals else None, _stream(stream)))
     return list(out) if totals else None
+
+
+def sample_chem(field, chem_index, stride=3, out=None, stream=None):
+    """sampleChem: every stride-th voxel of a [D][H][W] FP32 device field times
+    the chem's display scale, clamped at 1 -> [D//stride][H//stride][W//stride]."""
+    import torch
+    D, H, W = field.shape
+    if out is None:
+        out = torch.empty((D // stride, H // stride, W // stride), dtype=torch.float32, device=field.device)
+    check(lib.vfd_sample_chem(_ptr(field), D, H, W, int(chem_index), int(stride), _ptr(out), _stream(stream)))
+    return out
