@@ -35,7 +35,7 @@ template <typename T, int Q, int V = 0> struct RowCfg {
     static constexpr int NTT = var_ntmax(V) / 2;                    // target threads
     static constexpr int G = (NTT / TPF) > 0 ? (NTT / TPF) : 1;     // rows per group
     static constexpr int NT = TPF * G;
-    static constexpr int PITCH = Q + Q / 8 + 1;                  // padded LDS row
+    static constexpr int PITCH = Q + Q / 32 + 1;                 // padded LDS row (RowLds)
     // twiddle table W_Px (2Q entries) staged in LDS when it fits beside the rows
     static constexpr bool TW_LDS = (2 * Q + G * PITCH) * (int)sizeof(cx<T>) <= 64 * 1024;
     static constexpr int TW_N = TW_LDS ? 2 * Q : 1;
@@ -68,7 +68,9 @@ template <typename T, int TC> struct ColLds {
 };
 template <typename T> struct RowLds {
     cx<T>* base;  // row start
-    __device__ __forceinline__ cx<T>& at(int n) { return base[n + (n >> 3)]; }
+    // one pad element per 32: the Stockham stage stores (stride NS*R) and
+    // loads (stride Q/R) of the row transforms then spread over the banks
+    __device__ __forceinline__ cx<T>& at(int n) { return base[n + (n >> 5)]; }
 };
 
 // ------------------------------------------------------------- x forward
