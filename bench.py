@@ -22,6 +22,8 @@ Other workloads (one JSON line each, same fields where they apply):
                        plain per-field form (one field at a time, per-pass
                        launches, no graph).
   --workload copy      1 GiB device-to-device copy: measured HBM bandwidth.
+  --workload update    the post-diffusion chem update (WHChemical::update) of the
+                       8 fields on the device: levels, cleared deltas, 3 gradients.
   --workload precision 8 fields on 1536x768x192, log-uniform background in
                        [1e-6, 1e3] plus 500 sharp sources: the FP32 path against
                        the full-FP64 GPU path over 100 ticks (max-abs, rel-L2,
@@ -59,7 +61,8 @@ def parse():
                     help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
     ap.add_argument("--kspec", default="separable", choices=["separable", "real"],
                     help="kernel-spectrum format: FP64 1D tables formed in the z pass, or the 3D real array")
-    ap.add_argument("--workload", default="step", choices=["step", "ensemble", "serve", "copy", "precision"])
+    ap.add_argument("--workload", default="step",
+                    choices=["step", "ensemble", "serve", "copy", "precision", "update"])
     ap.add_argument("--ticks", type=int, default=100, help="precision: ticks of FP32 and FP64 side by side")
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
@@ -248,6 +251,52 @@ def bench_precision(args):
     # the drift against FP64 grows linearly with ticks: report it per tick too
     out["max_mass_drift_per_tick"] = out["max_mass_drift"] / args.ticks
     out["max_rel_l2_per_tick"] = worst / args.ticks
+    print(json.dumps(out), flush=True)
+
+
+def bench_update(args):
+    """vfd_chem_update over the 8 fields of the tissue-scale grid."""
+    import torch
+    import vfd
+    shape = tuple(int(s) for s in args.shape.split(","))
+    n = shape[0] * shape[1] * shape[2]
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(args.seed)
+    t = [torch.rand(n, generator=g, device="cuda:0") for _ in range(8)]
+    d = [torch.rand(n, generator=g, device="cuda:0") * 0.01 for _ in range(8)]
+    fc, fe = torch.rand(n, generator=g, device="cuda:0"), torch.rand(n, generator=g, device="cuda:0")
+    grads = [torch.empty(n, device="cuda:0") for _ in range(3)]
+    decay = [0.319, 0.00003052, 0.67, 0.2, 0.2, 0.7937, 0.9284, 0.2]
+    for _ in range(args.warmup):
+        vfd.chem_update(t, d, t, decay, fc, fe, grads, totals=False)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        vfd.chem_update(t, d, t, decay, fc, fe, grads, totals=False)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    t0 = time.perf_counter()
+    vfd.chem_update(t, d, t, decay, fc, fe, grads, totals=True)
+    with_totals_ms = (time.perf_counter() - t0) * 1e3
+    bytes_per_call = 148.0 * n       # reads 8 t + 8 d + 2 fragments, writes 8 p + 8 d + 3 gradients (FP32)
+    gbs = bytes_per_call / (ms * 1e-3) / 1e9
+    out = _header(args, "voxel-updates/s of the post-diffusion chem update (8 chems, levels + gradients + totals)",
+                  n / (ms * 1e-3), "voxel-updates/s", ms,
+                  {"workload": f"WHChemical::update (gpuUpdate) of 8 FP32 fields on {shape[2]}x{shape[1]}x{shape[0]}, "
+                               "in place (p aliases t)"})
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", f"hbm_traffic_chem_update_{shape[2]}x{shape[1]}x{shape[0]}.json")) as f:
+            b = json.load(f)["per_launch_bytes"]["chem_update"]
+            traffic = float(b["read"] + b["write"])
+    except (OSError, ValueError, KeyError):
+        pass
+    out["roofline"] = {"bound": "hbm", "kernel": "chem_update_kernel", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                       "algorithmic_bytes_per_call": bytes_per_call}
+    out["with_totals_host_ms"] = with_totals_ms
     print(json.dumps(out), flush=True)
 
 
@@ -464,6 +513,8 @@ def main():
         return bench_copy(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "update":
+        return bench_update(args)
     shape = tuple(int(s) for s in args.shape.split(","))
     nz, ny, nx = shape
     dev = "cuda:0"
