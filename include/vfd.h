@@ -59,7 +59,8 @@ typedef struct vfd_plan vfd_plan;
 
 typedef struct {
     int nz, ny, nx;  /* data box */
-    int pz, py, px;  /* FFT box; 0 means 2*n (zero-padded linear convolution) */
+    int pz, py, px;  /* FFT box; 0 picks the smallest compiled length >= 2*n (a
+                        zero-padded linear convolution for kernel radius <= P - n) */
     int precision;   /* VFD_F32 or VFD_F64 */
     int pad_mode;    /* VFD_PAD_ZERO or VFD_PAD_CONSTANT_VF */
     int kspec_format;/* VFD_KSPEC_REAL, VFD_KSPEC_COMPLEX_STD or VFD_KSPEC_SEPARABLE */

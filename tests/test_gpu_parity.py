@@ -240,3 +240,59 @@ def test_cpu_reference_workload_8_fields(vfd, orc):
         assert _report(f"field {f}", got, refs[f]) <= REL_F32
         mref = orc.neumaier_sum(refs[f])
         assert abs(mass[f] - mref) / mref <= MASS_F32
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 2), (1, 4, 6), (3, 1, 8), (2, 3, 2), (5, 7, 3), (1, 2, 40), (64, 1, 1),
+                                   (1, 1, 1)])
+def test_degenerate_grids(vfd, orc, shape):
+    """Grids with 1- or 2-voxel axes and odd extents: the default FFT box is
+    the smallest compiled length >= 2n, and a kernel radius up to P - n per axis
+    still gives the linear convolution."""
+    import torch
+    print(f"case degenerate shape={shape}", flush=True)
+    for precision in (4, 8):
+        plan = vfd.Plan(shape, precision=precision, kspec_format=vfd.KSPEC_SEPARABLE)
+        box = plan.box
+        assert all(p >= 2 * n for p, n in zip(box, shape))
+        r_max = min(p - n for p, n in zip(box, shape))
+        s2 = 0.95 * (r_max / 6.0) ** 2
+        assert orc.gauss_radius(s2) <= r_max
+        u = orc.synth_field(shape, 77, nsrc=1)
+        ref = orc.direct_separable(u, s2, 0.97)
+        dt = torch.float32 if precision == 4 else torch.float64
+        ud = torch.from_numpy(u).to(dt).cuda()
+        out = torch.empty_like(ud)
+        plan.diffuse([{"in": ud, "out": out, "work": plan.new_work(), "kspec": plan.gauss_spectrum(s2, 0.97)}])
+        torch.cuda.synchronize()
+        assert _report(f"{shape} prec {precision}", out.cpu().numpy(), ref) <= (REL_F32 if precision == 4 else REL_F64)
+
+
+def test_kernel_radius_beyond_padding_is_refused(vfd):
+    """A Gaussian whose radius exceeds the zero padding would wrap around:
+    the spectrum builder refuses it instead of returning a circular result."""
+    plan = vfd.Plan((8, 8, 12), box=(12, 16, 24), kspec_format=vfd.KSPEC_SEPARABLE)
+    with pytest.raises(vfd.VfdError) as ei:
+        plan.gauss_spectrum(4.0, 0.99)      # radius ceil(6*2) = 12 > 12 - 8 on z
+    assert ei.value.code == vfd.VFD_ERR_INVALID and "padding" in str(ei.value)
+
+
+def test_more_than_32_fields_and_empty_batch(vfd, orc):
+    """Batches above the 32 fields of one launch run in chunks; an empty batch
+    is a no-op."""
+    import torch
+    shape = (4, 6, 8)
+    print(f"case many_fields shape={shape}", flush=True)
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    plan.diffuse([])
+    ents, refs = [], []
+    for f in range(40):
+        u = orc.synth_field(shape, 500 + f, nsrc=2)
+        s2 = 0.1 + 0.005 * f          # radius ceil(6 sigma) <= 4 = P - n on z
+        refs.append(orc.direct_separable(u, s2, 0.99))
+        ud = torch.from_numpy(u).cuda()
+        ents.append({"in": ud, "out": torch.empty_like(ud), "work": plan.new_work(),
+                     "kspec": plan.gauss_spectrum(s2, 0.99)})
+    plan.diffuse(ents)
+    torch.cuda.synchronize()
+    for f in range(40):
+        assert orc.rel_l2(ents[f]["out"].cpu().numpy(), refs[f]) <= REL_F32, f

@@ -401,35 +401,45 @@ template <> struct BufOps<float> {
     }
 };
 // FP64: global addresses with 64-bit byte offsets, so one array may exceed
-// 4 GiB (the FP64 work array of a 1536x768x192 grid is 7.4 GB); a masked lane
-// passes OOB and its access is skipped (a load yields 0).
+// 4 GiB (the FP64 work array of a 1536x768x192 grid is 7.4 GB).  The handle
+// carries the array size: like a buffer resource, an access at or beyond it
+// (a masked lane passes OOB) is skipped and a load yields 0.
+struct GlobalSpan {
+    char* p;
+    unsigned long bytes;
+};
 template <> struct BufOps<double> {
-    using H = char*;
+    using H = GlobalSpan;
     using O = unsigned long;
     static constexpr O OOB = ~0ul;
-    static __device__ __forceinline__ H make(const void* p, unsigned long) { return (char*)const_cast<void*>(p); }
+    static __device__ __forceinline__ H make(const void* p, unsigned long bytes) {
+        return GlobalSpan{(char*)const_cast<void*>(p), bytes};
+    }
+    static __device__ __forceinline__ bool in(const H& h, O off, unsigned long n) {
+        return off < h.bytes && h.bytes - off >= n;
+    }
     typedef double f64x2 __attribute__((ext_vector_type(2)));
     template <int A = kCacheDefault> static __device__ __forceinline__ double2 ld_cx(H h, O off) {
-        if (off == OOB) return double2{0.0, 0.0};
-        const f64x2* q = (const f64x2*)(h + off);
+        if (!in(h, off, 16)) return double2{0.0, 0.0};
+        const f64x2* q = (const f64x2*)(h.p + off);
         if constexpr (A == kStream) return __builtin_bit_cast(double2, __builtin_nontemporal_load(q));
         return __builtin_bit_cast(double2, *q);
     }
     template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx(double2 v, H h, O off) {
-        if (off == OOB) return;
-        f64x2* q = (f64x2*)(h + off);
+        if (!in(h, off, 16)) return;
+        f64x2* q = (f64x2*)(h.p + off);
         if constexpr (A == kStream) __builtin_nontemporal_store(__builtin_bit_cast(f64x2, v), q);
         else *q = __builtin_bit_cast(f64x2, v);
     }
     template <int A = kCacheDefault> static __device__ __forceinline__ double ld_r(H h, O off) {
-        if (off == OOB) return 0.0;
-        const double* q = (const double*)(h + off);
+        if (!in(h, off, 8)) return 0.0;
+        const double* q = (const double*)(h.p + off);
         if constexpr (A == kStream) return __builtin_nontemporal_load(q);
         return *q;
     }
     template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(double v, H h, O off) {
-        if (off == OOB) return;
-        double* q = (double*)(h + off);
+        if (!in(h, off, 8)) return;
+        double* q = (double*)(h.p + off);
         if constexpr (A == kStream) __builtin_nontemporal_store(v, q);
         else *q = v;
     }

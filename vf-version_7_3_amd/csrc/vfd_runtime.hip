@@ -124,13 +124,28 @@ static long env_long(const char* name, long dflt) {
     return (s && *s) ? strtol(s, nullptr, 10) : dflt;
 }
 
+// Smallest FFT length >= 2n that a pass can run: even with a compiled half
+// length for x (R2C via the half-length complex FFT); for y and z a compiled
+// half length (even/odd split) or a compiled full length.
+static int default_box(int n, bool x_axis) {
+    if (n < 1) return 2 * n;
+    for (int p = 2 * n; p <= 2 * n + 4096; ++p) {
+        if (x_axis) {
+            if (p % 2 == 0 && length_supported(p / 2)) return p;
+        } else if ((p % 2 == 0 && length_supported(p / 2)) || length_supported(p)) {
+            return p;
+        }
+    }
+    return 2 * n;
+}
+
 int plan_init(const vfd_plan_desc* d, Plan* p) {
     if (!d) return set_error(VFD_ERR_INVALID, "plan: null descriptor");
     Plan q{};
     q.nz = d->nz; q.ny = d->ny; q.nx = d->nx;
-    q.pz = d->pz ? d->pz : 2 * d->nz;
-    q.py = d->py ? d->py : 2 * d->ny;
-    q.px = d->px ? d->px : 2 * d->nx;
+    q.pz = d->pz ? d->pz : default_box(d->nz, false);
+    q.py = d->py ? d->py : default_box(d->ny, false);
+    q.px = d->px ? d->px : default_box(d->nx, true);
     q.prec = d->precision;
     q.pad_mode = d->pad_mode;
     q.kspec_format = d->kspec_format;
@@ -475,8 +490,11 @@ int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspe
                          "gauss spectrum: plan kernel-spectrum format must be VFD_KSPEC_REAL or VFD_KSPEC_SEPARABLE");
     if (!(sigma2 > 0.0) || !d_kspec) return set_error(VFD_ERR_INVALID, "gauss spectrum: sigma2 must be > 0");
     const int R = (int)std::ceil(6.0 * std::sqrt(sigma2));
-    if (R >= p.px || R >= p.py || R >= p.pz)
-        return set_error(VFD_ERR_INVALID, "gauss spectrum: kernel radius %d does not fit the FFT box", R);
+    if (R > p.px - p.nx || R > p.py - p.ny || R > p.pz - p.nz)
+        return set_error(VFD_ERR_INVALID,
+                         "gauss spectrum: kernel radius %d exceeds the zero padding (FFT box %dx%dx%d, data %dx%dx%d): "
+                         "the product would wrap around instead of being a linear convolution",
+                         R, p.px, p.py, p.pz, p.nx, p.ny, p.nz);
     std::vector<double> gx, gy, gz;
     gauss_dft(sigma2, p.px, p.hx, gx);
     gauss_dft(sigma2, p.py, p.py, gy);
