@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
+    ap.add_argument("--slots", type=int, default=4, help="serve: batches in flight")
     return ap.parse_args()
 
 
@@ -387,7 +388,7 @@ def bench_serve(args):
         _fill_run(eng, rid, shape, nf, args.seed + 104729 * i)
         runs.append(rid)
         vox_of[rid] = float(np.prod(shape)) * nf
-    eng.serve_start(slots=2, max_batch_fields=256)
+    eng.serve_start(slots=args.slots, max_batch_fields=256)
     # warmup: one request per run (captures first-use costs)
     for t in [eng.submit(r) for r in runs]:
         eng.wait(t)
@@ -424,7 +425,7 @@ def bench_serve(args):
                   {"workload": f"Poisson stream mean {args.rate}/s for {args.duration}s over resident runs of "
                                f"{nf} FP32 fields, grids from the ensemble set plus 1024x512x128, "
                                f"HBM budget {args.budget_gb} GB",
-                   "runs_admitted": len(runs), "runs_refused": refused})
+                   "runs_admitted": len(runs), "runs_refused": refused, "slots": args.slots})
     out["requests"] = n_req
     out["requests_per_s"] = n_req / wall
     out["latency_ms"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
