@@ -68,7 +68,6 @@ struct GraphRec {
     std::vector<Held> held;
     uint64_t last_use = 0;
 };
-struct Batch;
 
 struct Request {
     uint64_t ticket;
