@@ -22,6 +22,11 @@ Other workloads (one JSON line each, same fields where they apply):
                        plain per-field form (one field at a time, per-pass
                        launches, no graph).
   --workload copy      1 GiB device-to-device copy: measured HBM bandwidth.
+  --workload dropin    the reference-compatible drop-in tick: 8 fields in pinned
+                       host memory, constant-VF pad, R2C-layout complex spectra,
+                       through vfdDiffuseChems (copies overlapped with the
+                       diffusion) and through the serial per-chem
+                       H2D -> fftDiffuse3D -> D2H loop of the reference.
   --workload update    the post-diffusion chem update (WHChemical::update) of the
                        8 fields on the device: levels, cleared deltas, 3 gradients.
   --workload precision 8 fields on 1536x768x192, log-uniform background in
@@ -62,7 +67,7 @@ def parse():
     ap.add_argument("--kspec", default="separable", choices=["separable", "real"],
                     help="kernel-spectrum format: FP64 1D tables formed in the z pass, or the 3D real array")
     ap.add_argument("--workload", default="step",
-                    choices=["step", "ensemble", "serve", "copy", "precision", "update"])
+                    choices=["step", "ensemble", "serve", "copy", "precision", "update", "dropin"])
     ap.add_argument("--ticks", type=int, default=100, help="precision: ticks of FP32 and FP64 side by side")
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
@@ -301,6 +306,83 @@ def bench_update(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_dropin(args):
+    """The reference's per-tick flow through the reference-compatible C interface."""
+    import ctypes
+    import numpy as np
+    import torch
+    import vfd
+    from vfd import legacy
+    nz, ny, nx = (int(s) for s in args.shape.split(","))
+    nf = 8
+    FD, FH, FW = 2 * nz, 2 * ny, 2 * nx
+    cc = legacy.CCtx()
+    cc.DD, cc.DH, cc.DW = nz, ny, nx
+    cc.FFTD, cc.FFTH, cc.FFTW = FD, FH, FW
+    rng = np.random.default_rng(args.seed)
+    host = [torch.rand((nz, ny, nx), dtype=torch.float32).pin_memory() for _ in range(nf)]
+    dev = [torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda:0") for _ in range(nf)]
+    spec = []
+    for ic in range(nf):
+        # a Gaussian spectrum in the reference's complex R2C layout (values do not affect timing)
+        s2 = 2.0 * rng.uniform(0.5, 5.0)
+        k = [torch.exp(-2.0 * (np.pi ** 2) * s2 * (torch.fft.fftfreq(n, device="cuda:0") ** 2)) for n in (FD, FH)]
+        kx = torch.exp(-2.0 * (np.pi ** 2) * s2 * (torch.fft.rfftfreq(FW, device="cuda:0") ** 2))
+        sp = (k[0][:, None, None] * k[1][None, :, None] * kx[None, None, :]).to(torch.complex64).contiguous()
+        spec.append(sp)
+        cc.h_ibuffs[ic] = host[ic].data_ptr()
+        cc.h_obuffs[ic] = host[ic].data_ptr()
+        cc.d_data[ic] = dev[ic].data_ptr()
+        cc.d_kernelspectrum_h[ic] = sp.data_ptr()
+        cc.dev_id[ic] = 0
+    base = (ctypes.c_float * nf)(*[0.01 * (ic + 1) for ic in range(nf)])
+    epi = nx - 4
+    torch.cuda.synchronize()
+
+    def pipelined():
+        if not legacy.lib.vfdDiffuseChems(cc, nf, epi, base):
+            raise vfd.VfdError(-1, vfd.last_error())
+
+    def serial():
+        for ic in range(nf):
+            dev[ic].copy_(host[ic])
+            torch.cuda.synchronize()
+            if not legacy.lib.fftDiffuse3D(dev[ic].data_ptr(), spec[ic].data_ptr(), 0, 0, cc, epi,
+                                           ctypes.c_float(base[ic]), ic, 0):
+                raise vfd.VfdError(-1, vfd.last_error())
+            host[ic].copy_(dev[ic])
+            torch.cuda.synchronize()
+
+    def resident():
+        for ic in range(nf):
+            if not legacy.lib.fftDiffuse3D(dev[ic].data_ptr(), spec[ic].data_ptr(), 0, 0, cc, epi,
+                                           ctypes.c_float(base[ic]), ic, 0):
+                raise vfd.VfdError(-1, vfd.last_error())
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    K = max(1, args.steps // 10)
+    t_pipe, t_serial, t_res = timed(pipelined, K), timed(serial, K), timed(resident, K)
+    vox = float(nx * ny * nz) * nf
+    out = _header(args, "voxel-updates/s of the drop-in tick through the reference C interface (PCIe-inclusive)",
+                  vox / t_pipe, "voxel-updates/s", t_pipe * 1e3,
+                  {"workload": f"8 FP32 fields {nx}x{ny}x{nz} in pinned host memory, constant-VF pad (epi {epi}), "
+                               f"complex R2C-layout spectra on the {FW}x{FH}x{FD} box, one tick"})
+    out["vfdDiffuseChems"] = {"ms_per_tick": t_pipe * 1e3, "voxel_updates_s": vox / t_pipe}
+    out["serial_h2d_fftDiffuse3D_d2h"] = {"ms_per_tick": t_serial * 1e3, "voxel_updates_s": vox / t_serial}
+    out["device_resident_fftDiffuse3D"] = {"ms_per_tick": t_res * 1e3, "voxel_updates_s": vox / t_res}
+    out["pcie_bytes_per_tick"] = 2.0 * vox * 4
+    out["speedup_vs_serial"] = t_serial / t_pipe
+    print(json.dumps(out), flush=True)
+
+
 def bench_ensemble(args):
     """Batched (graph) vs plain per-field ticks over uneven resident runs."""
     import numpy as np
@@ -533,6 +615,8 @@ def main():
         return bench_precision(args)
     if args.workload == "update":
         return bench_update(args)
+    if args.workload == "dropin":
+        return bench_dropin(args)
     shape = tuple(int(s) for s in args.shape.split(","))
     nz, ny, nx = shape
     dev = "cuda:0"

@@ -37,6 +37,8 @@ lib.H2D.argtypes = [_i, ctypes.POINTER(CCtx), _i]
 lib.H2D.restype = None
 lib.D2H.argtypes = [_i, ctypes.POINTER(CCtx), _i]
 lib.D2H.restype = None
+lib.vfdDiffuseChems.argtypes = [CCtx, _i, _i, ctypes.POINTER(_f)]
+lib.vfdDiffuseChems.restype = ctypes.c_bool
 lib.vfdSetKernelPowerQuirk.argtypes = [_i]
 lib.vfdSetKernelPowerQuirk.restype = None
 for name, n in (("extractCenter3D", 8), ("unpadResult3D", 8), ("padKernel3D", 11),
