@@ -295,14 +295,43 @@ def bench_update(args):
     traffic = None
     try:
         with open(os.path.join(ROOT, "profiles", f"hbm_traffic_chem_update_{shape[2]}x{shape[1]}x{shape[0]}.json")) as f:
-            b = json.load(f)["per_launch_bytes"]["chem_update"]
-            traffic = float(b["read"] + b["write"])
+            pb = json.load(f)["per_launch_bytes"]["chem_update"]
+            traffic = float(pb["read"] + pb["write"])
     except (OSError, ValueError, KeyError):
         pass
     out["roofline"] = {"bound": "hbm", "kernel": "chem_update_kernel", "achieved": gbs, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
                        "algorithmic_bytes_per_call": bytes_per_call}
     out["with_totals_host_ms"] = with_totals_ms
+    # the whole tick: diffusion then the update, against the update fused into
+    # the x-inverse pass (vfd_diffuse_update), both without the totals download
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    u = [x.view(shape) for x in t]
+    tt = [torch.empty_like(x) for x in u]
+    ks = [plan.gauss_spectrum(0.3 + 0.1 * i, 0.99) for i in range(8)]
+    works = [plan.new_work() for _ in range(8)]
+    split = plan.field_array([{"in": u[i], "out": tt[i], "work": works[i], "kspec": ks[i]} for i in range(8)])
+    fused = plan.field_array([{"in": u[i], "out": u[i], "work": works[i], "kspec": ks[i]} for i in range(8)])
+    plan.autotune(8)
+    dv = [x.view(shape) for x in d]
+
+    def tick_split():
+        plan.diffuse(split)
+        vfd.chem_update(tt, d, t, decay, fc, fe, grads, totals=False)
+
+    def tick_fused():
+        plan.diffuse_update(fused, dv, decay, fc, fe, grads, totals=False)
+
+    for name, fn in (("tick_split_ms", tick_split), ("tick_fused_ms", tick_fused)):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(args.steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        out[name] = a.elapsed_time(b) / args.steps
     print(json.dumps(out), flush=True)
 
 

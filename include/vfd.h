@@ -152,6 +152,17 @@ int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_
                     const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,
                     double* totals, void* stream);
 
+/* One tick of the 8 chem fields with the chem update fused into it: the
+ * diffusion of vfd_diffuse, whose x-inverse pass writes fields[ic].out =
+ * d_delta[ic] + t * decay[ic] (t the diffused field, FP32 product rounded
+ * before the add) and clears d_delta[ic]; then the gradients and compensated
+ * totals of vfd_chem_update from those levels.  Bit-identical to vfd_diffuse
+ * followed by vfd_chem_update, with one pass over the fields fewer.  FP32
+ * plans; fields and deltas in the chem order of vfd_chem_update. */
+int vfd_diffuse_update(const vfd_plan* plan, const vfd_field* fields, float* const* d_delta, const float* decay,
+                       const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, double* totals,
+                       void* stream);
+
 /* Visualisation downsample: sampleChem (src/Visualization/3D/volumeRender_kernel.cu:
  * 215-322) into a dense array instead of a surface.  d_dst[dz][dy][dx] =
  * min(d_src[stride*dz][stride*dy][stride*dx] * scale[chem_index], 1) for

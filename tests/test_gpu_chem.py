@@ -61,3 +61,73 @@ def test_sample_chem_matches_oracle(vfd, orc, shape, stride):
     for chem in range(8):
         got = vfd.sample_chem(ud, chem, stride).cpu().numpy()
         assert np.array_equal(got, orc.sample_chem(u, chem, stride)), chem
+
+
+@pytest.mark.parametrize("shape", [(32, 48, 64), (20, 17, 33), (128, 512, 1024)])
+def test_fused_diffuse_update_bit_identical(vfd, shape):
+    """vfd_diffuse_update (update fused into the x-inverse epilogue) against
+    vfd_diffuse followed by vfd_chem_update on the same inputs: levels,
+    cleared deltas, gradients and totals all bit-identical."""
+    import torch
+    print(f"case fused_diffuse_update shape={shape}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(sum(shape))
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    decay = [0.319, 0.00003052, 0.67, 0.2, 0.2, 0.7937, 0.9284, 0.2]   # woundHealingWorld.cpp:2636-2643
+    u = [torch.rand(shape, generator=g, device="cuda") for _ in range(8)]
+    d = [torch.randn(shape, generator=g, device="cuda") * 0.01 for _ in range(8)]
+    fc, fe = torch.rand(shape, generator=g, device="cuda"), torch.rand(shape, generator=g, device="cuda")
+    ks = [plan.gauss_spectrum(0.3 + 0.1 * i, 0.99) for i in range(8)]
+    # reference order: diffuse into t, then the update p = d + t * decay
+    t = [torch.empty_like(x) for x in u]
+    plan.diffuse([{"in": u[i], "out": t[i], "work": plan.new_work(), "kspec": ks[i]} for i in range(8)])
+    d_ref = [x.clone() for x in d]
+    p_ref = [torch.empty_like(x) for x in u]
+    g_ref = [torch.empty_like(fc) for _ in range(3)]
+    tot_ref = vfd.chem_update(t, d_ref, p_ref, decay, fc, fe, g_ref)
+    # fused
+    d_fu = [x.clone() for x in d]
+    p_fu = [torch.empty_like(x) for x in u]
+    g_fu = [torch.empty_like(fc) for _ in range(3)]
+    tot_fu = plan.diffuse_update([{"in": u[i], "out": p_fu[i], "work": plan.new_work(), "kspec": ks[i]}
+                                  for i in range(8)], d_fu, decay, fc, fe, g_fu)
+    torch.cuda.synchronize()
+    for k in range(8):
+        assert torch.equal(p_fu[k], p_ref[k]), k
+        assert torch.equal(d_fu[k], d_ref[k]) and not d_fu[k].any(), k
+        assert tot_fu[k] == tot_ref[k], k
+    for j in range(3):
+        assert torch.equal(g_fu[j], g_ref[j]), j
+
+
+def test_fused_diffuse_update_in_place(vfd):
+    """The levels may overwrite the input fields (out aliases in), as the ABM's tick does."""
+    import torch
+    shape = (24, 40, 56)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    decay = [0.5] * 8
+    u = [torch.rand(shape, generator=g, device="cuda") for _ in range(8)]
+    d = [torch.rand(shape, generator=g, device="cuda") for _ in range(8)]
+    fc, fe = torch.rand(shape, generator=g, device="cuda"), torch.rand(shape, generator=g, device="cuda")
+    ks = [plan.gauss_spectrum(0.5, 0.97) for _ in range(8)]
+    t = [torch.empty_like(x) for x in u]
+    plan.diffuse([{"in": u[i], "out": t[i], "work": plan.new_work(), "kspec": ks[i]} for i in range(8)])
+    p_ref, g_ref = [torch.empty_like(x) for x in u], [torch.empty_like(fc) for _ in range(3)]
+    vfd.chem_update(t, [x.clone() for x in d], p_ref, decay, fc, fe, g_ref, totals=False)
+    g_fu = [torch.empty_like(fc) for _ in range(3)]
+    plan.diffuse_update([{"in": u[i], "out": u[i], "work": plan.new_work(), "kspec": ks[i]} for i in range(8)],
+                        d, decay, fc, fe, g_fu, totals=False)
+    torch.cuda.synchronize()
+    for k in range(8):
+        assert torch.equal(u[k], p_ref[k]), k
+    for j in range(3):
+        assert torch.equal(g_fu[j], g_ref[j]), j
+
+
+def test_fused_diffuse_update_refuses_bad_arguments(vfd):
+    import torch
+    plan64 = vfd.Plan((8, 8, 8), precision=vfd.F64)
+    z = torch.zeros((8, 8, 8), dtype=torch.float32, device="cuda")
+    ents = [{"in": z, "out": z, "work": plan64.new_work(), "kspec": plan64.new_kspec()} for _ in range(8)]
+    with pytest.raises(vfd.VfdError):
+        plan64.diffuse_update(ents, [z] * 8, [0.5] * 8, z, z, [z] * 3)

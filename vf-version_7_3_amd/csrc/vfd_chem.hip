@@ -45,6 +45,9 @@ __device__ __forceinline__ void nadd(double& s, double& c, double x) {
 
 // Each block owns one contiguous chunk of voxels; lanes stride through it and
 // a fixed-order tree combines the per-lane (sum, compensation) pairs.
+// LEVELS: the levels are already in cp.p (the x-inverse pass wrote them);
+// only the gradients and totals are formed.
+template <bool LEVELS>
 __global__ void __launch_bounds__(kThreads)
 chem_update_kernel(ChemPtrs cp, const float* __restrict__ frag_col, const float* __restrict__ frag_ela,
                    float* __restrict__ neu, float* __restrict__ mac, float* __restrict__ fib, long n, long chunk,
@@ -59,13 +62,19 @@ chem_update_kernel(ChemPtrs cp, const float* __restrict__ frag_col, const float*
         float lv[kChems];
 #pragma unroll
         for (int ic = 0; ic < kChems; ++ic) {
-            const float scaled = cp.t[ic][i] * cp.decay[ic];
-            lv[ic] = cp.d[ic][i] + scaled;
+            if constexpr (LEVELS) {
+                lv[ic] = cp.p[ic][i];
+            } else {
+                const float scaled = cp.t[ic][i] * cp.decay[ic];
+                lv[ic] = cp.d[ic][i] + scaled;
+            }
         }
 #pragma unroll
         for (int ic = 0; ic < kChems; ++ic) {
-            cp.d[ic][i] = 0.0f;
-            cp.p[ic][i] = lv[ic];
+            if constexpr (!LEVELS) {
+                cp.d[ic][i] = 0.0f;
+                cp.p[ic][i] = lv[ic];
+            }
             nadd(s[ic], c[ic], (double)lv[ic]);
         }
         // gradients (WHChemical.cpp:437-445), left-to-right FP32 sums
@@ -157,6 +166,54 @@ extern "C" int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem
     return hip_check(hipGetLastError(), "sample_chem launch");
 }
 
+// levels (unless already formed), gradients and totals for 8 chems of n voxels
+static int chem_pass(const ChemPtrs& cp, bool levels_in_p, const float* d_frag_col, const float* d_frag_ela,
+                     float* const* d_grads, long n, double* totals, hipStream_t st) {
+    const int nb = (int)std::min<long>(kBlocks, std::max<long>(1, (n + kThreads - 1) / kThreads));
+    const long chunk = (n + nb - 1) / nb;
+    double2* partial = nullptr;
+    double* d_tot = nullptr;
+    const size_t pbytes = (size_t)kChems * nb * sizeof(double2);
+    if (int rc = hip_check(hipMallocAsync((void**)&partial, pbytes + kChems * sizeof(double), st),
+                           "hipMallocAsync(chem update)"))
+        return rc;
+    d_tot = (double*)((char*)partial + pbytes);
+    if (levels_in_p)
+        hipLaunchKernelGGL(chem_update_kernel<true>, dim3(nb), dim3(kThreads), 0, st, cp, d_frag_col, d_frag_ela,
+                           d_grads[0], d_grads[1], d_grads[2], n, chunk, partial);
+    else
+        hipLaunchKernelGGL(chem_update_kernel<false>, dim3(nb), dim3(kThreads), 0, st, cp, d_frag_col, d_frag_ela,
+                           d_grads[0], d_grads[1], d_grads[2], n, chunk, partial);
+    int rc = hip_check(hipGetLastError(), "chem update launch");
+    if (!rc) {
+        hipLaunchKernelGGL(chem_total_kernel, dim3(kChems), dim3(kThreads), 0, st, partial, nb, d_tot);
+        rc = hip_check(hipGetLastError(), "chem totals launch");
+    }
+    if (!rc && totals) {
+        rc = hip_check(hipMemcpyAsync(totals, d_tot, kChems * sizeof(double), hipMemcpyDeviceToHost, st),
+                       "chem totals download");
+        if (!rc) rc = hip_check(hipStreamSynchronize(st), "chem totals sync");
+    }
+    (void)hipFreeAsync(partial, st);
+    return rc;
+}
+
+namespace vfd {
+// gradients and totals of 8 levels the x-inverse pass has already written
+int chem_grads_totals(float* const* d_p, const float* d_frag_col, const float* d_frag_ela, float* const* d_grads,
+                      long n, double* totals, hipStream_t st) {
+    if (!d_p || !d_frag_col || !d_frag_ela || !d_grads) return set_error(VFD_ERR_INVALID, "chem update: null argument");
+    ChemPtrs cp{};
+    for (int ic = 0; ic < kChems; ++ic) {
+        if (!d_p[ic]) return set_error(VFD_ERR_INVALID, "chem update: chem %d has a null field", ic);
+        cp.p[ic] = d_p[ic];
+    }
+    for (int g = 0; g < 3; ++g)
+        if (!d_grads[g]) return set_error(VFD_ERR_INVALID, "chem update: gradient %d is null", g);
+    return chem_pass(cp, true, d_frag_col, d_frag_ela, d_grads, n, totals, st);
+}
+}  // namespace vfd
+
 extern "C" int vfd_chem_update(const float* const* d_t, float* const* d_d, float* const* d_p, const float* decay,
                                const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, long n,
                                double* totals, void* stream) {
@@ -174,28 +231,5 @@ extern "C" int vfd_chem_update(const float* const* d_t, float* const* d_d, float
     }
     for (int g = 0; g < 3; ++g)
         if (!d_grads[g]) return set_error(VFD_ERR_INVALID, "chem_update: gradient %d is null", g);
-    hipStream_t st = (hipStream_t)stream;
-    const int nb = (int)std::min<long>(kBlocks, std::max<long>(1, (n + kThreads - 1) / kThreads));
-    const long chunk = (n + nb - 1) / nb;
-    double2* partial = nullptr;
-    double* d_tot = nullptr;
-    const size_t pbytes = (size_t)kChems * nb * sizeof(double2);
-    if (int rc = hip_check(hipMallocAsync((void**)&partial, pbytes + kChems * sizeof(double), st),
-                           "hipMallocAsync(chem update)"))
-        return rc;
-    d_tot = (double*)((char*)partial + pbytes);
-    hipLaunchKernelGGL(chem_update_kernel, dim3(nb), dim3(kThreads), 0, st, cp, d_frag_col, d_frag_ela,
-                       d_grads[0], d_grads[1], d_grads[2], n, chunk, partial);
-    int rc = hip_check(hipGetLastError(), "chem update launch");
-    if (!rc) {
-        hipLaunchKernelGGL(chem_total_kernel, dim3(kChems), dim3(kThreads), 0, st, partial, nb, d_tot);
-        rc = hip_check(hipGetLastError(), "chem totals launch");
-    }
-    if (!rc && totals) {
-        rc = hip_check(hipMemcpyAsync(totals, d_tot, kChems * sizeof(double), hipMemcpyDeviceToHost, st),
-                       "chem totals download");
-        if (!rc) rc = hip_check(hipStreamSynchronize(st), "chem totals sync");
-    }
-    (void)hipFreeAsync(partial, st);
-    return rc;
+    return chem_pass(cp, false, d_frag_col, d_frag_ela, d_grads, n, totals, (hipStream_t)stream);
 }

@@ -31,7 +31,12 @@ int plan_init(const vfd_plan_desc* d, Plan* p);
 void tuned_lookup(Plan& q);
 int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms);
 int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
-int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st);
+int chem_grads_totals(float* const* d_p, const float* d_frag_col, const float* d_frag_ela, float* const* d_grads,
+                      long n, double* totals, hipStream_t st);
+// deltas / decay (optional, FP32 plans): the x-inverse pass writes
+// out = deltas[f] + t * decay[f] and clears deltas[f] (the fused chem update)
+int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st,
+                        float* const* deltas = nullptr, const float* decay = nullptr);
 double plan_pass_bytes(const Plan& p, int pass);
 int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspec, hipStream_t st);
 int plan_mass(const Plan& p, const void* const* d_fields, int nfields, double* mass, hipStream_t st);

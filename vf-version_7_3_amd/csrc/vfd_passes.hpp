@@ -160,6 +160,15 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
     }
 }
 
+// d + t * decay with the product rounded to FP32 before the add (no fused
+// multiply-add), the level update of WHChemical::update
+__device__ __forceinline__ float upd_level(float d, float t, float decay) {
+#pragma clang fp contract(off)
+    const float scaled = t * decay;
+    return d + scaled;
+}
+__device__ __forceinline__ double upd_level(double d, double t, double decay) { return d + t * decay; }
+
 // ------------------------------------------------------------- x inverse
 // Z_k = (A_k + A*_{L-k}) + i (A_k - A*_{L-k}) W_Px^-k, length-L inverse FFT,
 // u(2n) = Re z_n, u(2n+1) = Im z_n (unnormalized C2R of length Px), crop to Nx,
@@ -218,6 +227,11 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         }
         const O orow = (O)(row * g.Nx) * RB;
         const bool vec = (g.Nx % 2 == 0);
+        // fused chem update (FP32): p = d + t * decay with the product rounded
+        // before the add (WHChemical.cpp:425-428), and d cleared
+        [[maybe_unused]] const bool upd = sizeof(T) == 4 && fd.delta != nullptr;
+        [[maybe_unused]] const auto rdel = B::make(fd.delta, g.out_bytes);
+        [[maybe_unused]] const T dec = (T)fd.decay;
 #pragma unroll
         for (int b = 0; b < Sh::NBL; ++b)
 #pragma unroll
@@ -226,12 +240,32 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 const cx<T> val = v[b * Sh::RL + r];
                 if (vec) {
                     const bool ok = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::template st_cx<kStream>(cx<T>{val.x + addc, val.y + addc}, rout, ok ? orow + (O)(2 * n) * RB : B::OOB);
+                    const O off = ok ? orow + (O)(2 * n) * RB : B::OOB;
+                    cx<T> o{val.x + addc, val.y + addc};
+                    if constexpr (sizeof(T) == 4) {
+                        if (upd) {
+                            const cx<T> d = B::ld_cx(rdel, off);
+                            o = cx<T>{upd_level(d.x, o.x, dec), upd_level(d.y, o.y, dec)};
+                            B::template st_cx<kStream>(cx<T>{T(0), T(0)}, rdel, off);
+                        }
+                    }
+                    B::template st_cx<kStream>(o, rout, off);
                 } else {
                     const bool ok0 = n >= 0 && live && 2 * n < g.Nx;
                     const bool ok1 = n >= 0 && live && 2 * n + 1 < g.Nx;
-                    B::template st_r<kStream>(val.x + addc, rout, ok0 ? orow + (O)(2 * n) * RB : B::OOB);
-                    B::template st_r<kStream>(val.y + addc, rout, ok1 ? orow + (O)(2 * n + 1) * RB : B::OOB);
+                    const O off0 = ok0 ? orow + (O)(2 * n) * RB : B::OOB;
+                    const O off1 = ok1 ? orow + (O)(2 * n + 1) * RB : B::OOB;
+                    T o0 = val.x + addc, o1 = val.y + addc;
+                    if constexpr (sizeof(T) == 4) {
+                        if (upd) {
+                            o0 = upd_level(B::ld_r(rdel, off0), o0, dec);
+                            o1 = upd_level(B::ld_r(rdel, off1), o1, dec);
+                            B::template st_r<kStream>(T(0), rdel, off0);
+                            B::template st_r<kStream>(T(0), rdel, off1);
+                        }
+                    }
+                    B::template st_r<kStream>(o0, rout, off0);
+                    B::template st_r<kStream>(o1, rout, off1);
                 }
             }
         // the next item's first LDS store happens after a barrier inside fft_regs

@@ -247,7 +247,7 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
 // ------------------------------------------------------------- execution
 template <typename T>
 static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pass_lo, int pass_hi,
-                    hipStream_t st) {
+                    hipStream_t st, float* const* deltas, const float* decay) {
     for (int f0 = 0; f0 < nfields; f0 += kMaxFieldsPerLaunch) {
         const int nf = std::min(kMaxFieldsPerLaunch, nfields - f0);
         FieldBatch fb;
@@ -260,6 +260,8 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             fb.f[i].kspec = f.kspec;
             fb.f[i].base = f.base;
             fb.f[i].epi = f.epi;
+            fb.f[i].delta = deltas ? deltas[f0 + i] : nullptr;
+            fb.f[i].decay = decay ? decay[f0 + i] : 0.0f;
         }
         GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD,
                    p.field_bytes, p.work_bytes, p.kspec_bytes, p.field_bytes};
@@ -302,12 +304,13 @@ static int check_fields(const vfd_field* fields, int nfields) {
     return VFD_OK;
 }
 
-int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st) {
+int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st,
+                        float* const* deltas, const float* decay) {
     if (int rc = check_fields(fields, nfields)) return rc;
     if (lo < 0 || hi > 5 || lo >= hi) return set_error(VFD_ERR_INVALID, "diffuse: bad pass range [%d,%d)", lo, hi);
     if (nfields == 0) return VFD_OK;
-    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, lo, hi, st)
-                             : run_tick<double>(p, fields, nfields, lo, hi, st);
+    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, lo, hi, st, deltas, decay)
+                             : run_tick<double>(p, fields, nfields, lo, hi, st, deltas, decay);
 }
 
 double plan_pass_bytes(const Plan& p, int pass) {
@@ -686,6 +689,21 @@ int vfd_plan_get_info(const vfd_plan* plan, vfd_plan_info* info) {
 int vfd_diffuse(const vfd_plan* plan, const vfd_field* fields, int nfields, void* stream) {
     if (!plan) return set_error(VFD_ERR_INVALID, "diffuse: null plan");
     return plan_diffuse(plan->p, fields, nfields, (hipStream_t)stream);
+}
+
+int vfd_diffuse_update(const vfd_plan* plan, const vfd_field* fields, float* const* d_delta, const float* decay,
+                       const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, double* totals,
+                       void* stream) {
+    if (!plan || !fields || !d_delta || !decay) return set_error(VFD_ERR_INVALID, "diffuse_update: null argument");
+    if (plan->p.prec != VFD_F32) return set_error(VFD_ERR_UNSUPPORTED, "diffuse_update: FP32 plans only");
+    for (int ic = 0; ic < 8; ++ic)
+        if (!d_delta[ic]) return set_error(VFD_ERR_INVALID, "diffuse_update: chem %d has no delta field", ic);
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = plan_diffuse_passes(plan->p, fields, 8, 0, 5, st, d_delta, decay)) return rc;
+    float* levels[8];
+    for (int ic = 0; ic < 8; ++ic) levels[ic] = (float*)fields[ic].out;
+    const long n = (long)plan->p.nx * plan->p.ny * plan->p.nz;
+    return chem_grads_totals(levels, d_frag_col, d_frag_ela, d_grads, n, totals, st);
 }
 
 int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream) {

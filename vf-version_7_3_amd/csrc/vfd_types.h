@@ -18,6 +18,9 @@ struct FieldDesc {
     double base;        // constant pad value (PAD_CONSTANT_VF), else 0
     int epi;            // epithelium index: interior x < epi (PAD_CONSTANT_VF)
     int pad_;
+    float* delta;       // fused chem update: out = delta + t * decay, delta = 0 (null: plain output)
+    float decay;
+    int pad2_;
 };
 
 // Fields of one launch, passed by value as a kernel argument (graph-capture

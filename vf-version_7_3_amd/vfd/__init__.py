@@ -70,6 +70,8 @@ lib.vfd_plan_set_schedule.argtypes = [_p, ctypes.POINTER(_i)]
 lib.vfd_gauss_spectrum.argtypes = [_p, _d, _d, _p, _p]
 lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _p]
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
+lib.vfd_diffuse_update.argtypes = [_p, ctypes.POINTER(Field), ctypes.POINTER(_p), ctypes.POINTER(ctypes.c_float),
+                                   _p, _p, ctypes.POINTER(_p), ctypes.POINTER(_d), _p]
 lib.vfd_sample_chem.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
 lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
                                 ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
@@ -166,6 +168,21 @@ class Plan:
     def diffuse_pass(self, fields, pass_index, stream=None):
         arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
         check(lib.vfd_diffuse_pass(self.h, arr, len(arr), int(pass_index), _stream(stream)))
+
+    def diffuse_update(self, fields, deltas, decay, frag_col, frag_ela, grads, stream=None, totals=True):
+        """One tick of the 8 chem fields with the chem update fused: fields[i]['out']
+        receives deltas[i] + t * decay[i], deltas are cleared, grads (NEU, MAC, FIB)
+        and the 8 totals are formed.  Returns the totals unless totals=False."""
+        arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
+        if len(arr) != 8 or len(deltas) != 8 or len(decay) != 8 or len(grads) != 3:
+            raise ValueError("diffuse_update: 8 fields, 8 deltas, 8 decays and 3 gradient fields")
+        dv = (ctypes.c_void_p * 8)(*[x.data_ptr() for x in deltas])
+        gv = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in grads])
+        dec = (ctypes.c_float * 8)(*[float(x) for x in decay])
+        out = (ctypes.c_double * 8)()
+        check(lib.vfd_diffuse_update(self.h, arr, dv, dec, _ptr(frag_col), _ptr(frag_ela), gv,
+                                     out if totals else None, _stream(stream)))
+        return list(out) if totals else None
 
     def pass_bytes(self, pass_index):
         return lib.vfd_plan_pass_bytes(self.h, int(pass_index))
