@@ -27,6 +27,8 @@ Other workloads (one JSON line each, same fields where they apply):
                        through vfdDiffuseChems (copies overlapped with the
                        diffusion) and through the serial per-chem
                        H2D -> fftDiffuse3D -> D2H loop of the reference.
+                       --shape 1006,1390,110 --box 1152,1536,256 is the ABM's
+                       production configuration (HUMAN_VF).
   --workload update    the post-diffusion chem update (WHChemical::update) of the
                        8 fields on the device: levels, cleared deltas, 3 gradients.
   --workload precision 8 fields on 1536x768x192, log-uniform background in
@@ -59,6 +61,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="128,512,1024", help="nz,ny,nx of the data box")
+    ap.add_argument("--box", default=None, help="dropin: FFT box FD,FH,FW (default twice the data box)")
     ap.add_argument("--fields", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -344,7 +347,7 @@ def bench_dropin(args):
     from vfd import legacy
     nz, ny, nx = (int(s) for s in args.shape.split(","))
     nf = 8
-    FD, FH, FW = 2 * nz, 2 * ny, 2 * nx
+    FD, FH, FW = (int(s) for s in args.box.split(",")) if args.box else (2 * nz, 2 * ny, 2 * nx)
     cc = legacy.CCtx()
     cc.DD, cc.DH, cc.DW = nz, ny, nx
     cc.FFTD, cc.FFTH, cc.FFTW = FD, FH, FW

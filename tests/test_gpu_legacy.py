@@ -239,3 +239,77 @@ def test_convolution_clamp_cpu(vfd, leg):
     k2[1, 0] = 1.0   # picks data[y][x+1] with clamp
     leg.lib.convolutionClampToBorderCPU(res.ctypes.data, data.ctypes.data, k2.ctypes.data, 3, 4, 3, 3, 1, 1)
     assert list(res[0]) == [1, 2, 3, 3]
+
+
+def _gauss_dft(g, P, nk):
+    """DFT over a length-P box of the centred symmetric window g (length 2R+1):
+    G[k] = g0 + 2 sum_j g_j cos(2 pi j k / P), k < nk, FP64."""
+    R = (len(g) - 1) // 2
+    k = np.arange(nk, dtype=np.float64)[:, None]
+    j = np.arange(1, R + 1, dtype=np.float64)[None, :]
+    return g[R] + 2.0 * (np.cos(2.0 * np.pi * j * k / P) * g[R + 1:][None, :]).sum(axis=1)
+
+
+def test_fftDiffuse3D_production_shape(vfd, orc, leg):
+    """The ABM's production configuration through the reference C interface:
+    a 110x1390x1006 (x, y, z) chem field on the 256x1536x1152 FFT box,
+    constant-VF pad (epithelium at x = 106, baseChem on the faces and beyond),
+    complex R2C-layout spectrum of a sigma = 15.3-patch Gaussian (radius 92,
+    the production kernel's width).  Sub-boxes at a corner of the baseChem
+    region, around a source, and at the far faces are checked against an FP64
+    direct convolution of the padded field with the oracle's window."""
+    import torch
+    D, H, W = 1006, 1390, 110
+    box = (1152, 1536, 256)
+    epi, base, decay = 106, 0.37, 0.98
+    sigma2 = 15.3 ** 2
+    print(f"case fftDiffuse3D production D,H,W={D},{H},{W} box={box}", flush=True)
+    g = orc.gauss_window1d(sigma2)
+    R = (len(g) - 1) // 2
+    assert R <= min(box[0] - D, box[1] - H, box[2] - W)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(17)
+    u = torch.exp(torch.randn((D, H, W), generator=gen, device="cuda") - 2.0)
+    src = [(500, 700, 50), (120, 300, 20), (900, 1200, 100)]
+    for z, y, x in src:
+        u[z, y, x] += 5000.0
+    ud = u.clone()
+    gz = torch.from_numpy(_gauss_dft(g, box[0], box[0])).cuda()
+    gy = torch.from_numpy(_gauss_dft(g, box[1], box[1])).cuda()
+    gx = torch.from_numpy(_gauss_dft(g, box[2], box[2] // 2 + 1)).cuda()
+    ks = ((decay * gz)[:, None, None] * gy[None, :, None] * gx[None, None, :]).to(torch.complex64)
+    cc = leg.CCtx()
+    cc.DD, cc.DH, cc.DW = D, H, W
+    cc.FFTD, cc.FFTH, cc.FFTW = box
+    cc.KD = cc.KH = cc.KW = 2 * R + 1
+    cc.KZ = cc.KY = cc.KX = R
+    pf, pi = ctypes.c_int(), ctypes.c_int()
+    assert leg.lib.vfdPlan3d(ctypes.byref(pf), *box, 0x2a) == 0
+    assert leg.lib.vfdPlan3d(ctypes.byref(pi), *box, 0x2c) == 0
+    assert leg.lib.fftDiffuse3D(ctypes.c_void_p(ud.data_ptr()), ctypes.c_void_p(ks.data_ptr()), pf, pi, cc,
+                                epi, ctypes.c_float(base), 0, 0)
+    torch.cuda.synchronize()
+    assert leg.lib.vfdDestroy(pf) == 0 and leg.lib.vfdDestroy(pi) == 0
+    del ks
+
+    def expected(lo, n):
+        # padded field on [lo - R, lo + n + R) per axis: the field inside
+        # 0<z<D-1, 0<y<H-1, 0<x<epi, baseChem everywhere else
+        rng = [np.arange(l - R, l + m + R) for l, m in zip(lo, n)]
+        inside = [(r > 0) & (r < lim) for r, lim in zip(rng, (D - 1, H - 1, epi))]
+        cl = [np.clip(r, 0, lim - 1) for r, lim in zip(rng, (D, H, W))]
+        blk = u[cl[0][0]:cl[0][-1] + 1, cl[1][0]:cl[1][-1] + 1, cl[2][0]:cl[2][-1] + 1].double().cpu().numpy()
+        blk = blk[np.ix_(cl[0] - cl[0][0], cl[1] - cl[1][0], cl[2] - cl[2][0])]
+        mask = inside[0][:, None, None] & inside[1][None, :, None] & inside[2][None, None, :]
+        a = np.where(mask, blk, base)
+        for ax in (2, 1, 0):                           # x, y, z: valid part of the correlation
+            m = a.shape[ax] - 2 * R
+            a = sum(g[j] * np.take(a, np.arange(j, j + m), axis=ax) for j in range(2 * R + 1))
+        return decay * a
+
+    for lo, n in [((0, 0, 96), (12, 12, 14)), ((494, 694, 44), (12, 12, 12)),
+                  ((994, 1378, 0), (12, 12, 12)), ((114, 294, 14), (12, 12, 12))]:
+        ref = expected(lo, n)
+        got = ud[lo[0]:lo[0] + n[0], lo[1]:lo[1] + n[1], lo[2]:lo[2] + n[2]].double().cpu().numpy()
+        print(f"  sub-box at {lo}:", end="", flush=True)
+        assert _rel(got, ref) <= 1e-5

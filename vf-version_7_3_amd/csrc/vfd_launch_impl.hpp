@@ -106,9 +106,22 @@ template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                          const void* tw, hipStream_t st) {
     using C = ColCfg<T, Q, V>;
-    dim3 grid((unsigned)((cd.ncols + C::TC - 1) / C::TC), (unsigned)nplanes, (unsigned)nfields);
-    hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), grid, dim3(C::NT), 0, st, fields, cd,
-                       (const cx<T>*)tw);
+    if constexpr (C::SH > 1) {
+        // grouped 1-D launch (col_kernel): 8 segment groups per 8 * SH ids
+        ColDesc g = cd;
+        g.grid_segs = (unsigned)((cd.ncols + C::TC_LINE - 1) / C::TC_LINE);
+        g.grid_planes = (unsigned)nplanes;
+        g.grid_fields = (unsigned)nfields;
+        const unsigned long groups = (unsigned long)g.grid_segs * nplanes * nfields;
+        const unsigned long ids = (groups + 7) / 8 * 8 * C::SH;
+        if (ids > 0x7fffffffu) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), dim3((unsigned)ids), dim3(C::NT), 0, st, fields, g,
+                           (const cx<T>*)tw);
+    } else {
+        dim3 grid((unsigned)((cd.ncols + C::TC - 1) / C::TC), (unsigned)nplanes, (unsigned)nfields);
+        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), grid, dim3(C::NT), 0, st, fields, cd,
+                           (const cx<T>*)tw);
+    }
     return hipGetLastError();
 }
 

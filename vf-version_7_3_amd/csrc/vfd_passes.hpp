@@ -57,6 +57,8 @@ template <typename T, int Q, int V = 0> struct ColCfg {
     }
     static constexpr int TC = tc_for();  // columns per block
     static constexpr int NT = TC * TPF;
+    // tiles sharing one 128-byte row segment (narrow tiles of long columns)
+    static constexpr int SH = TC < TC_LINE ? TC_LINE / TC : 1;
     // twiddle table W_P (P = Q * rs <= 2Q entries) staged in LDS
     static constexpr int TW_N = 2 * Q;
 };
@@ -301,15 +303,35 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     __shared__ cx<T> tw[C::TW_N];
     constexpr bool SEP = MODE == COL_FUSED && KS == KS_SEP;
     __shared__ double gzs[SEP ? P : 1];  // separable: the gz table of the z frequencies
-    const FieldDesc& fd = fb.f[blockIdx.z];
+    // Block coordinates.  Narrow tiles (SH of them share each 128-byte row
+    // segment) launch as a 1-D grid in which the SH tiles of one segment get
+    // work-group ids 8 apart.  Ids are dealt round-robin to the 8 XCDs, so the
+    // tiles sharing a line run side by side on one XCD and its L2 fetches the
+    // line once.
+    unsigned bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if constexpr (C::SH > 1) {
+        const unsigned lin = blockIdx.x;
+        const unsigned grp = (lin / (8u * C::SH)) * 8u + lin % 8u;
+        const unsigned seg = grp % cd.grid_segs, rest = grp / cd.grid_segs;
+        bx = seg * C::SH + (lin / 8u) % C::SH;
+        by = rest % cd.grid_planes;
+        bz = rest / cd.grid_planes;
+        // whole-block exits, before any barrier: padding ids and tiles past the row end
+        if (bz >= cd.grid_fields || (long)bx * C::TC >= cd.ncols) return;
+    }
+    const FieldDesc& fd = fb.f[bz];
     const auto rwork = B::make(fd.work, cd.work_bytes);
     const int c = threadIdx.x % C::TC;
     const int t = threadIdx.x / C::TC;
-    const long col = (long)blockIdx.x * C::TC + c;
+    const long col = (long)bx * C::TC + c;
     const bool live = col < cd.ncols;
-    const O base = (O)((long)blockIdx.y * cd.pstride + col) * CB;
+    const O base = (O)((long)by * cd.pstride + col) * CB;
     const O rstep = (O)cd.rstride * CB;
     ColLds<T, C::TC> lds{smem + c};
+    // non-temporal loads only when a tile row spans whole 128-byte lines: a
+    // narrower tile (long columns) shares each line with its neighbours, which
+    // must still find it in the cache
+    constexpr int NTP = C::TC * (int)CB >= 128 ? kStream : kCacheDefault;
     constexpr int NI = ShF::NB0 * ShF::R0;
     constexpr int NO = ShI::NBL * ShI::RL;
     constexpr int NK = ShF::NBL * ShF::RL;
@@ -326,7 +348,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShF::R0; ++r) {
                 const int n = ShF::in_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nin && live;
-                x[b * ShF::R0 + r] = B::template ld_cx<kStream>(rwork, ok ? base + (O)n * rstep : B::OOB);
+                x[b * ShF::R0 + r] = B::template ld_cx<NTP>(rwork, ok ? base + (O)n * rstep : B::OOB);
             }
     }
     if constexpr (MODE == COL_INV) {
@@ -361,14 +383,14 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
         using OD = typename BD::O;
         const auto tk = BD::make(fd.kspec, cd.kspec_bytes);
         const double sc = BD::ld_r(tk, OD(0));
-        const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + blockIdx.y) * 8u);
+        const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + by) * 8u);
         const double gxv = BD::ld_r(tk, live ? (OD)(1 + col) * 8u : BD::OOB);
         ka = (gxv * gyv) * sc;
         for (int i = threadIdx.x; i < P; i += C::NT) gzs[i] = BD::ld_r(tk, (OD)(cd.sep_gz + i) * 8u);
     } else if constexpr (MODE == COL_FUSED) {
         const auto rk = B::make(fd.kspec, cd.kspec_bytes);
         if constexpr (KS == KS_COMPLEX) {
-            const O kbase = (O)((long)blockIdx.y * cd.ks_plane + col) * CB;
+            const O kbase = (O)((long)by * cd.ks_plane + col) * CB;
             const O kstep = (O)cd.ks_row * CB;
 #pragma unroll
             for (int b = 0; b < ShF::NBL; ++b)
@@ -382,7 +404,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 }
         } else {
             static_assert(KS == KS_REAL, "streamed spectrum format");
-            const O kbase = (O)((long)blockIdx.y * cd.ks_plane + col * RS) * RB;
+            const O kbase = (O)((long)by * cd.ks_plane + col * RS) * RB;
             const O kstep = (O)cd.ks_row * RB;
 #pragma unroll
             for (int b = 0; b < ShF::NBL; ++b)

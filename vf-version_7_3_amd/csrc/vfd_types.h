@@ -56,6 +56,9 @@ struct ColDesc {
     int sep_gy, sep_gz; // table offsets (elements) of gy and gz
     double scale;       // applied to K^ products (1 / (Px Py Pz))
     unsigned long work_bytes, kspec_bytes;  // per-field array sizes
+    // grouped 1-D launch of narrow tiles (set by the launcher): planes and
+    // fields of the launch, 128-byte row segments per plane row
+    unsigned grid_planes, grid_fields, grid_segs, pad_;
 };
 
 // Every per-field array is addressed with 32-bit byte offsets.
