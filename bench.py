@@ -392,7 +392,8 @@ def bench_dropin(args):
                 raise vfd.VfdError(-1, vfd.last_error())
 
     def timed(fn, reps):
-        fn()
+        for _ in range(max(1, args.warmup)):
+            fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):

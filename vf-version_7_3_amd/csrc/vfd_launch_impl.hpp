@@ -9,11 +9,12 @@
 // box and its half).
 #define VFD_COLD_LENGTHS(X)                                                                    \
     X(2) X(4) X(8) X(16) X(2048) X(3) X(6) X(12) X(24) X(48) X(96) X(9) X(18) X(36) X(72)      \
-    X(144) X(288) X(576) X(1152) X(2304) X(616) X(308) X(5) X(10) X(20) X(40) X(80) X(160)     \
+    X(144) X(288) X(576) X(2304) X(616) X(308) X(5) X(10) X(20) X(40) X(80) X(160)     \
     X(320) X(640) X(1280)
-// lengths of the tissue-scale, ensemble and precision configurations: every
+// lengths of the tissue-scale, ensemble, precision and production (1152 x 1536
+// z-y box) configurations: every
 // schedule variant is compiled for these and the autotuner picks one
-#define VFD_HOT_LENGTHS(X) X(32) X(64) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+#define VFD_HOT_LENGTHS(X) X(32) X(64) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1152) X(1536)
 
 namespace vfd {
 
@@ -105,7 +106,7 @@ hipError_t launch_xinv<VFD_REAL>(int L, const FieldBatch& fields, int nfields, c
 template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                          const void* tw, hipStream_t st) {
-    using C = ColCfg<T, Q, V>;
+    using C = ColCfg<T, Q, V, RS>;
     if constexpr (C::SH > 1) {
         // grouped 1-D launch (col_kernel): 8 segment groups per 8 * SH ids
         ColDesc g = cd;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -257,6 +258,43 @@ bool fail(int rc) {
 
 bool hip_ok(hipError_t e, const char* what) { return hip_check(e, what) == VFD_OK || fail(-1); }
 
+// The persistent workspace of a (device, data box, FFT box): the plan and one
+// work array, made on first use.  The plan's schedule is autotuned then (about
+// a second, once per box and process) unless VFD_SCHEDULE fixes it or
+// VFD_AUTOTUNE=0 keeps the default.
+int ws_for(int dev, const c_ctx& cctx, Ws** out) {
+    const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
+    auto it = g_ws.find(key);
+    if (it == g_ws.end()) {
+        vfd_plan_desc d{cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW, VFD_F32, VFD_PAD_CONSTANT_VF,
+                        VFD_KSPEC_COMPLEX_STD};
+        Ws ws;
+        if (int rc = plan_init(&d, &ws.plan)) return rc;
+        if (int rc = hip_check(hipMalloc(&ws.work, ws.plan.work_bytes), "legacy workspace")) return rc;
+        const char* fixed = getenv("VFD_SCHEDULE");
+        const char* tune = getenv("VFD_AUTOTUNE");
+        if (!fixed && !(tune && atoi(tune) == 0)) {
+            // on a stream of its own: work on the null stream would tie the
+            // copy/compute pipeline's streams to it
+            hipStream_t ts = nullptr;
+            int rc = hip_check(hipStreamCreateWithFlags(&ts, hipStreamNonBlocking), "autotune stream");
+            if (!rc) rc = plan_autotune(ws.plan, 1, ts, nullptr);
+            if (ts) (void)hipStreamDestroy(ts);
+            if (rc) {
+                (void)hipFree(ws.work);
+                return rc;
+            }
+        }
+        if (getenv("VFD_VERBOSE"))
+            fprintf(stderr, "vfd: workspace %dx%dx%d in %dx%dx%d, schedule %d,%d,%d,%d,%d\n", cctx.DW, cctx.DH, cctx.DD,
+                    cctx.FFTW, cctx.FFTH, cctx.FFTD, ws.plan.var[0], ws.plan.var[1], ws.plan.var[2], ws.plan.var[3],
+                    ws.plan.var[4]);
+        it = g_ws.emplace(key, ws).first;
+    }
+    *out = &it->second;
+    return VFD_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -376,18 +414,10 @@ bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlan
     }
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
-    const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
-    auto it = g_ws.find(key);
-    if (it == g_ws.end()) {
-        vfd_plan_desc d{cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW, VFD_F32, VFD_PAD_CONSTANT_VF,
-                        VFD_KSPEC_COMPLEX_STD};
-        Ws ws;
-        if (int rc = plan_init(&d, &ws.plan)) return fail(rc);
-        if (!hip_ok(hipMalloc(&ws.work, ws.plan.work_bytes), "fftDiffuse3D workspace")) return false;
-        it = g_ws.emplace(key, ws).first;
-    }
-    vfd_field f{d_Data, d_Data, it->second.work, d_KernelSpectrum, (double)baseChem, epiBoundary, 0};
-    if (int rc = plan_diffuse(it->second.plan, &f, 1, nullptr)) return fail(rc);
+    Ws* ws = nullptr;
+    if (int rc = ws_for(dev, cctx, &ws)) return fail(rc);
+    vfd_field f{d_Data, d_Data, ws->work, d_KernelSpectrum, (double)baseChem, epiBoundary, 0};
+    if (int rc = plan_diffuse(ws->plan, &f, 1, nullptr)) return fail(rc);
     return hip_ok(hipStreamSynchronize(nullptr), "fftDiffuse3D");
 }
 
@@ -435,16 +465,8 @@ extern "C" bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const fl
     if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
-    const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
-    auto it = g_ws.find(key);
-    if (it == g_ws.end()) {
-        vfd_plan_desc d{cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW, VFD_F32, VFD_PAD_CONSTANT_VF,
-                        VFD_KSPEC_COMPLEX_STD};
-        Ws ws;
-        if (int rc = plan_init(&d, &ws.plan)) return fail(rc);
-        if (!hip_ok(hipMalloc(&ws.work, ws.plan.work_bytes), "vfdDiffuseChems workspace")) return false;
-        it = g_ws.emplace(key, ws).first;
-    }
+    Ws* ws = nullptr;
+    if (int rc = ws_for(dev, cctx, &ws)) return fail(rc);
     Pipe* pp = nullptr;
     if (int rc = pipe_for(dev, &pp)) return fail(rc);
     const size_t bytes = (size_t)cctx.DD * cctx.DH * cctx.DW * sizeof(float);
@@ -460,8 +482,8 @@ extern "C" bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const fl
         if (!rc) rc = hip_check(hipEventRecord(pp->e_in[ic], pp->in), "pipeline event");
         if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_in[ic], 0), "pipeline wait");
         if (!rc) {
-            vfd_field f{d, d, it->second.work, cctx.d_kernelspectrum_h[ic], (double)baseChem[ic], epiBoundary, 0};
-            rc = plan_diffuse(it->second.plan, &f, 1, pp->comp);
+            vfd_field f{d, d, ws->work, cctx.d_kernelspectrum_h[ic], (double)baseChem[ic], epiBoundary, 0};
+            rc = plan_diffuse(ws->plan, &f, 1, pp->comp);
         }
         if (!rc) rc = hip_check(hipEventRecord(pp->e_comp[ic], pp->comp), "pipeline event");
         if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_comp[ic], 0), "pipeline wait");

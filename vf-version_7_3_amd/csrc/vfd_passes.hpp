@@ -41,14 +41,14 @@ template <typename T, int Q, int V = 0> struct RowCfg {
     static constexpr int TW_N = TW_LDS ? 2 * Q : 1;
 };
 
-template <typename T, int Q, int V = 0> struct ColCfg {
+template <typename T, int Q, int V = 0, int RS = 2> struct ColCfg {
     static constexpr int MAXR = var_maxr(V);
     static constexpr int TPF = tpf_for(Q, MAXR);
     static constexpr int LDS_CAP = 72 * 1024;
     // columns per block: the variant's thread cap, but never narrower than one
     // 128-byte line of a row (narrower tiles fetch whole lines and use part)
     static constexpr int TC_LINE = 128 / (int)sizeof(cx<T>);
-    static constexpr bool lds_fits(int tc) { return (tc * Q + 2 * Q) * (int)sizeof(cx<T>) <= LDS_CAP; }
+    static constexpr bool lds_fits(int tc) { return (tc * Q + Q * RS) * (int)sizeof(cx<T>) <= LDS_CAP; }
     static constexpr int tc_for() {
         int tc = 64;
         while (tc > 1 && (tc * TPF > var_ntmax(V) || !lds_fits(tc))) tc /= 2;
@@ -59,8 +59,8 @@ template <typename T, int Q, int V = 0> struct ColCfg {
     static constexpr int NT = TC * TPF;
     // tiles sharing one 128-byte row segment (narrow tiles of long columns)
     static constexpr int SH = TC < TC_LINE ? TC_LINE / TC : 1;
-    // twiddle table W_P (P = Q * rs <= 2Q entries) staged in LDS
-    static constexpr int TW_N = 2 * Q;
+    // twiddle table W_P (P = Q * RS entries) staged in LDS
+    static constexpr int TW_N = Q * RS;
 };
 
 // ------------------------------------------------------------ LDS views
@@ -290,9 +290,9 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
 enum { KS_REAL = 0, KS_COMPLEX = 1, KS_SEP = 2 };
 
 template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
-__global__ void __launch_bounds__((ColCfg<T, Q, V>::NT))
+__global__ void __launch_bounds__((ColCfg<T, Q, V, RS>::NT))
 col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
-    using C = ColCfg<T, Q, V>;
+    using C = ColCfg<T, Q, V, RS>;
     using ShF = FftShape<T, Q, C::TPF, false, C::MAXR>;                // forward / inverse-only plan
     using ShI = FftShape<T, Q, C::TPF, MODE == COL_FUSED, C::MAXR>;    // inverse plan
     using B = BufOps<T>;
