@@ -126,13 +126,12 @@ static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& c
     return hipGetLastError();
 }
 
-// fused pass: the spectrum format picks the instantiation (the R2C-layout
-// complex format of the reference interface keeps the default schedule)
+// fused pass: the spectrum format picks the instantiation
 template <typename T, int Q, int RS, int V>
 static hipError_t col_fused(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                             const void* tw, hipStream_t st) {
     if (cd.kspec_sep) return col_go<T, Q, COL_FUSED, RS, V, KS_SEP>(fields, nfields, cd, nplanes, tw, st);
-    if (cd.kspec_complex) return col_go<T, Q, COL_FUSED, RS, 0, KS_COMPLEX>(fields, nfields, cd, nplanes, tw, st);
+    if (cd.kspec_complex) return col_go<T, Q, COL_FUSED, RS, V, KS_COMPLEX>(fields, nfields, cd, nplanes, tw, st);
     return col_go<T, Q, COL_FUSED, RS, V, KS_REAL>(fields, nfields, cd, nplanes, tw, st);
 }
 
