@@ -171,6 +171,15 @@ int vfd_diffuse_update(const vfd_plan* plan, const vfd_field* fields, float* con
 int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem_index, int stride, float* d_dst,
                     void* stream);
 
+/* vfd_diffuse with the visualisation downsample of vfd_sample_chem fused into
+ * the x-inverse pass (the unpad epilogue, where the reference samples the
+ * diffused field: Diffusion.cu:1878-1903): d_vis[i] receives the downsample of
+ * fields[i]'s diffused output with chem chem_index[i]'s display scale.
+ * Bit-identical to vfd_diffuse followed by vfd_sample_chem on each output.
+ * FP32 plans; stride >= 1.  Asynchronous. */
+int vfd_diffuse_sample(const vfd_plan* plan, const vfd_field* fields, int nfields, const int* chem_index, int stride,
+                       float* const* d_vis, void* stream);
+
 /* ======================================================================
  * Engine: resident runs, batched ticks, serving.
  *

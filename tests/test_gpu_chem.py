@@ -131,3 +131,41 @@ def test_fused_diffuse_update_refuses_bad_arguments(vfd):
     ents = [{"in": z, "out": z, "work": plan64.new_work(), "kspec": plan64.new_kspec()} for _ in range(8)]
     with pytest.raises(vfd.VfdError):
         plan64.diffuse_update(ents, [z] * 8, [0.5] * 8, z, z, [z] * 3)
+
+
+@pytest.mark.parametrize("shape,stride", [((32, 48, 64), 3), ((20, 17, 33), 2), ((7, 9, 10), 4), ((128, 512, 1024), 3)])
+def test_fused_sample_bit_identical(vfd, shape, stride):
+    """vfd_diffuse_sample (the sampleChem downsample fused into the x-inverse
+    epilogue) against vfd_diffuse followed by vfd_sample_chem: the diffused
+    fields and the downsampled arrays bit-identical, for even and odd nx."""
+    import torch
+    print(f"case fused_sample shape={shape} stride={stride}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(sum(shape) + stride)
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    nf = 3
+    u = [torch.rand(shape, generator=g, device="cuda") * 1e-4 for _ in range(nf)]
+    ks = [plan.gauss_spectrum(0.4 + 0.2 * i, 0.99) for i in range(nf)]
+    chems = [0, 3, 7]
+    ref_out = [torch.empty_like(x) for x in u]
+    plan.diffuse([{"in": u[i], "out": ref_out[i], "work": plan.new_work(), "kspec": ks[i]} for i in range(nf)])
+    ref_vis = [vfd.sample_chem(ref_out[i], chems[i], stride) for i in range(nf)]
+    out = [torch.empty_like(x) for x in u]
+    vis = [torch.full_like(r, -1.0) for r in ref_vis]
+    plan.diffuse_sample([{"in": u[i], "out": out[i], "work": plan.new_work(), "kspec": ks[i]} for i in range(nf)],
+                        chems, stride, vis)
+    torch.cuda.synchronize()
+    for i in range(nf):
+        assert torch.equal(out[i], ref_out[i]), i
+        assert torch.equal(vis[i], ref_vis[i]), i
+    assert any(bool((r == 1.0).any()) for r in ref_vis) or shape[0] < 16  # the clamp is exercised
+
+
+def test_fused_sample_refuses_bad_arguments(vfd):
+    import torch
+    plan = vfd.Plan((8, 8, 8))
+    z = torch.zeros((8, 8, 8), dtype=torch.float32, device="cuda")
+    ent = [{"in": z, "out": z, "work": plan.new_work(), "kspec": plan.new_kspec()}]
+    with pytest.raises(vfd.VfdError):
+        plan.diffuse_sample(ent, [8], 2, [z])
+    with pytest.raises(vfd.VfdError):
+        plan.diffuse_sample(ent, [0], 0, [z])

@@ -158,11 +158,9 @@ extern "C" int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem
     const int sD = D / stride, sH = H / stride, sW = W / stride;
     const long total = (long)sD * sH * sW;
     if (total == 0) return VFD_OK;
-    // display scales per chem (volumeRender_kernel.cu:258-294)
-    static const float scale[kChems] = {50000.0f, 10000.0f, 1000000.0f, 100.0f, 10000.0f, 100000.0f, 100000.0f, 10000.0f};
     const unsigned blocks = (unsigned)std::min<long>(4096, (total + 255) / 256);
     hipLaunchKernelGGL(sample_chem_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d_src, H, W, sD, sH, sW,
-                       stride, scale[chem_index], d_dst);
+                       stride, vfd::chem_display_scale(chem_index), d_dst);
     return hip_check(hipGetLastError(), "sample_chem launch");
 }
 
@@ -199,6 +197,12 @@ static int chem_pass(const ChemPtrs& cp, bool levels_in_p, const float* d_frag_c
 }
 
 namespace vfd {
+// display scales per chem (volumeRender_kernel.cu:258-294)
+float chem_display_scale(int chem) {
+    static const float scale[kChems] = {50000.0f, 10000.0f, 1000000.0f, 100.0f, 10000.0f, 100000.0f, 100000.0f, 10000.0f};
+    return scale[chem];
+}
+
 // gradients and totals of 8 levels the x-inverse pass has already written
 int chem_grads_totals(float* const* d_p, const float* d_frag_col, const float* d_frag_ela, float* const* d_grads,
                       long n, double* totals, hipStream_t st) {

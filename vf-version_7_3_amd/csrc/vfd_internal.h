@@ -31,12 +31,22 @@ int plan_init(const vfd_plan_desc* d, Plan* p);
 void tuned_lookup(Plan& q);
 int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms);
 int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
+float chem_display_scale(int chem);  // sampleChem's per-chem display scale, chem < 8
 int chem_grads_totals(float* const* d_p, const float* d_frag_col, const float* d_frag_ela, float* const* d_grads,
                       long n, double* totals, hipStream_t st);
-// deltas / decay (optional, FP32 plans): the x-inverse pass writes
-// out = deltas[f] + t * decay[f] and clears deltas[f] (the fused chem update)
+// Optional work of the x-inverse epilogue (FP32 plans), per field f:
+//  deltas/decay: out = deltas[f] + t * decay[f], deltas[f] cleared (the chem update);
+//  vis/vis_scale/vis_stride: the visualisation downsample min(t * vis_scale[f], 1)
+//  of every vis_stride-th voxel into vis[f].
+struct Epilogue {
+    float* const* deltas = nullptr;
+    const float* decay = nullptr;
+    float* const* vis = nullptr;
+    const float* vis_scale = nullptr;
+    int vis_stride = 0;
+};
 int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st,
-                        float* const* deltas = nullptr, const float* decay = nullptr);
+                        const Epilogue* epi = nullptr);
 double plan_pass_bytes(const Plan& p, int pass);
 int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspec, hipStream_t st);
 int plan_mass(const Plan& p, const void* const* d_fields, int nfields, double* mass, hipStream_t st);

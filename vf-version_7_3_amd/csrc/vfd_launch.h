@@ -18,7 +18,7 @@ hipError_t launch_xfwd(int L, const FieldBatch& fields, int nfields, const GridD
 // x-inverse (C2R + unpad).
 template <typename T>
 hipError_t launch_xinv(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
-                       const void* tw, hipStream_t st, int variant = 0);
+                       const void* tw, hipStream_t st, int variant = 0, bool vis = false);
 // column pass of sub-length Q (P = Q * cd.rs) over nplanes planes.
 template <typename T>
 hipError_t launch_col(int Q, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,

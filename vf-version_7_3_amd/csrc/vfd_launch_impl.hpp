@@ -42,13 +42,13 @@ static int blocks_per_cu(K kernel, int nt) {
 }
 
 #ifdef VFD_ROW_TU
-template <typename T, int Q, int V, bool FWD>
+template <typename T, int Q, int V, bool FWD, bool VIS = false>
 static hipError_t row_go(const FieldBatch& fields, int nfields, const GridDesc& g, const void* tw,
                          hipStream_t st) {
     using C = RowCfg<T, Q, V>;
     const long rows = (long)g.Ny * g.Nz;
     const long items = (rows + C::G - 1) / C::G * nfields;
-    auto kern = FWD ? xfwd_kernel<T, Q, V> : xinv_kernel<T, Q, V>;
+    auto kern = FWD ? xfwd_kernel<T, Q, V> : xinv_kernel<T, Q, V, VIS>;
     const long cap = (long)num_cus() * blocks_per_cu(kern, C::NT);
     const unsigned grid = (unsigned)(items < cap ? items : cap);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(C::NT), 0, st, fields, g, (const cx<T>*)tw, nfields);
@@ -95,9 +95,27 @@ hipError_t launch_xfwd<VFD_REAL>(int L, const FieldBatch& fields, int nfields, c
     return row_dispatch<true>(L, variant, fields, nfields, g, tw, st);
 }
 
+// x inverse with the fused visualisation downsample: FP32, default schedule
+static hipError_t xinv_vis(int L, const FieldBatch& fields, int nfields, const GridDesc& g, const void* tw,
+                           hipStream_t st) {
+    if constexpr (sizeof(VFD_REAL) == 4) {
+        switch (L) {
+#define VFD_ANY(Q) \
+    case Q: return row_go<VFD_REAL, Q, 0, false, true>(fields, nfields, g, tw, st);
+            VFD_HOT_LENGTHS(VFD_ANY)
+            VFD_COLD_LENGTHS(VFD_ANY)
+#undef VFD_ANY
+            default:
+                return hipErrorInvalidValue;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 template <>
 hipError_t launch_xinv<VFD_REAL>(int L, const FieldBatch& fields, int nfields, const GridDesc& g,
-                                 const void* tw, hipStream_t st, int variant) {
+                                 const void* tw, hipStream_t st, int variant, bool vis) {
+    if (vis) return xinv_vis(L, fields, nfields, g, tw, st);
     return row_dispatch<false>(L, variant, fields, nfields, g, tw, st);
 }
 #endif

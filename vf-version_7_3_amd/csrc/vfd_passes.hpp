@@ -175,7 +175,7 @@ __device__ __forceinline__ double upd_level(double d, double t, double decay) { 
 // Z_k = (A_k + A*_{L-k}) + i (A_k - A*_{L-k}) W_Px^-k, length-L inverse FFT,
 // u(2n) = Re z_n, u(2n+1) = Im z_n (unnormalized C2R of length Px), crop to Nx,
 // plus base * K^(0,0,0) in constant-pad mode.
-template <typename T, int L, int V>
+template <typename T, int L, int V, bool VIS = false>
 __global__ void __launch_bounds__((RowCfg<T, L, V>::NT))
 xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int nfields) {
     using C = RowCfg<T, L, V>;
@@ -234,6 +234,22 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         [[maybe_unused]] const bool upd = sizeof(T) == 4 && fd.delta != nullptr;
         [[maybe_unused]] const auto rdel = B::make(fd.delta, g.out_bytes);
         [[maybe_unused]] const T dec = (T)fd.decay;
+        // fused visualisation downsample of t (FP32): sampleChem's every
+        // vs-th voxel, min(t * scale, 1), into the dense [Nz/vs][Ny/vs][Nx/vs] array
+        const bool vis = VIS && fd.vis != nullptr;
+        const int vs = vis ? fd.vis_stride : 1;
+        const int vD = g.Nz / vs, vH = g.Ny / vs, vW = g.Nx / vs;
+        const bool vrow = vis && live && z % vs == 0 && y % vs == 0 && z / vs < vD && y / vs < vH;
+        const auto rvis = B::make(fd.vis, vis ? (unsigned long)vD * vH * vW * RB : 0ul);
+        const O vbase = vrow ? (O)(((long)(z / vs) * vH + y / vs) * vW) * RB : (O)0;
+        const float vscale = fd.vis_scale;
+        auto vis_put = [&](int x, T tv) {
+            if (x % vs == 0 && x / vs < vW) {
+                float s = (float)tv * vscale;
+                if (s > 1.0f) s = 1.0f;
+                B::st_r((T)s, rvis, vbase + (O)(x / vs) * RB);
+            }
+        };
 #pragma unroll
         for (int b = 0; b < Sh::NBL; ++b)
 #pragma unroll
@@ -244,6 +260,12 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                     const bool ok = n >= 0 && live && 2 * n + 1 < g.Nx;
                     const O off = ok ? orow + (O)(2 * n) * RB : B::OOB;
                     cx<T> o{val.x + addc, val.y + addc};
+                    if constexpr (VIS) {
+                        if (vrow && ok) {
+                            vis_put(2 * n, o.x);
+                            vis_put(2 * n + 1, o.y);
+                        }
+                    }
                     if constexpr (sizeof(T) == 4) {
                         if (upd) {
                             const cx<T> d = B::ld_cx(rdel, off);
@@ -258,6 +280,10 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                     const O off0 = ok0 ? orow + (O)(2 * n) * RB : B::OOB;
                     const O off1 = ok1 ? orow + (O)(2 * n + 1) * RB : B::OOB;
                     T o0 = val.x + addc, o1 = val.y + addc;
+                    if constexpr (VIS) {
+                        if (vrow && ok0) vis_put(2 * n, o0);
+                        if (vrow && ok1) vis_put(2 * n + 1, o1);
+                    }
                     if constexpr (sizeof(T) == 4) {
                         if (upd) {
                             o0 = upd_level(B::ld_r(rdel, off0), o0, dec);

@@ -247,7 +247,7 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
 // ------------------------------------------------------------- execution
 template <typename T>
 static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pass_lo, int pass_hi,
-                    hipStream_t st, float* const* deltas, const float* decay) {
+                    hipStream_t st, const Epilogue* epi) {
     for (int f0 = 0; f0 < nfields; f0 += kMaxFieldsPerLaunch) {
         const int nf = std::min(kMaxFieldsPerLaunch, nfields - f0);
         FieldBatch fb;
@@ -260,8 +260,15 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             fb.f[i].kspec = f.kspec;
             fb.f[i].base = f.base;
             fb.f[i].epi = f.epi;
-            fb.f[i].delta = deltas ? deltas[f0 + i] : nullptr;
-            fb.f[i].decay = decay ? decay[f0 + i] : 0.0f;
+            if (epi && epi->deltas) {
+                fb.f[i].delta = epi->deltas[f0 + i];
+                fb.f[i].decay = epi->decay[f0 + i];
+            }
+            if (epi && epi->vis) {
+                fb.f[i].vis = epi->vis[f0 + i];
+                fb.f[i].vis_scale = epi->vis_scale[f0 + i];
+                fb.f[i].vis_stride = epi->vis_stride;
+            }
         }
         GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD,
                    p.field_bytes, p.work_bytes, p.kspec_bytes, p.field_bytes};
@@ -288,7 +295,9 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
                 case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st, p.var[pass]); break;
                 case VFD_PASS_ZCONV: e = launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[pass]); break;
                 case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
-                case VFD_PASS_XINV: e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
+                case VFD_PASS_XINV:
+                    e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass], epi && epi->vis);
+                    break;
             }
             if (e != hipSuccess) return hip_check(e, "pass launch");
         }
@@ -305,12 +314,12 @@ static int check_fields(const vfd_field* fields, int nfields) {
 }
 
 int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int lo, int hi, hipStream_t st,
-                        float* const* deltas, const float* decay) {
+                        const Epilogue* epi) {
     if (int rc = check_fields(fields, nfields)) return rc;
     if (lo < 0 || hi > 5 || lo >= hi) return set_error(VFD_ERR_INVALID, "diffuse: bad pass range [%d,%d)", lo, hi);
     if (nfields == 0) return VFD_OK;
-    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, lo, hi, st, deltas, decay)
-                             : run_tick<double>(p, fields, nfields, lo, hi, st, deltas, decay);
+    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, lo, hi, st, epi)
+                             : run_tick<double>(p, fields, nfields, lo, hi, st, epi);
 }
 
 double plan_pass_bytes(const Plan& p, int pass) {
@@ -699,11 +708,34 @@ int vfd_diffuse_update(const vfd_plan* plan, const vfd_field* fields, float* con
     for (int ic = 0; ic < 8; ++ic)
         if (!d_delta[ic]) return set_error(VFD_ERR_INVALID, "diffuse_update: chem %d has no delta field", ic);
     hipStream_t st = (hipStream_t)stream;
-    if (int rc = plan_diffuse_passes(plan->p, fields, 8, 0, 5, st, d_delta, decay)) return rc;
+    Epilogue ep;
+    ep.deltas = d_delta;
+    ep.decay = decay;
+    if (int rc = plan_diffuse_passes(plan->p, fields, 8, 0, 5, st, &ep)) return rc;
     float* levels[8];
     for (int ic = 0; ic < 8; ++ic) levels[ic] = (float*)fields[ic].out;
     const long n = (long)plan->p.nx * plan->p.ny * plan->p.nz;
     return chem_grads_totals(levels, d_frag_col, d_frag_ela, d_grads, n, totals, st);
+}
+
+int vfd_diffuse_sample(const vfd_plan* plan, const vfd_field* fields, int nfields, const int* chem_index, int stride,
+                       float* const* d_vis, void* stream) {
+    if (!plan || !fields || !chem_index || !d_vis) return set_error(VFD_ERR_INVALID, "diffuse_sample: null argument");
+    if (plan->p.prec != VFD_F32) return set_error(VFD_ERR_UNSUPPORTED, "diffuse_sample: FP32 plans only");
+    if (stride < 1) return set_error(VFD_ERR_INVALID, "diffuse_sample: stride %d", stride);
+    if (nfields < 0) return set_error(VFD_ERR_INVALID, "diffuse_sample: negative field count");
+    std::vector<float> scale(nfields);
+    for (int i = 0; i < nfields; ++i) {
+        if (chem_index[i] < 0 || chem_index[i] >= 8)
+            return set_error(VFD_ERR_INVALID, "diffuse_sample: chem index %d", chem_index[i]);
+        if (!d_vis[i]) return set_error(VFD_ERR_INVALID, "diffuse_sample: field %d has no output array", i);
+        scale[i] = chem_display_scale(chem_index[i]);
+    }
+    Epilogue ep;
+    ep.vis = d_vis;
+    ep.vis_scale = scale.data();
+    ep.vis_stride = stride;
+    return plan_diffuse_passes(plan->p, fields, nfields, 0, 5, (hipStream_t)stream, &ep);
 }
 
 int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream) {

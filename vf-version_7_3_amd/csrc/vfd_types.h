@@ -20,6 +20,9 @@ struct FieldDesc {
     int pad_;
     float* delta;       // fused chem update: out = delta + t * decay, delta = 0 (null: plain output)
     float decay;
+    int vis_stride;     // fused visualisation downsample of t (vis non-null): every vis_stride-th voxel
+    float* vis;         //   min(t * vis_scale, 1) into a dense [Nz/s][Ny/s][Nx/s] array
+    float vis_scale;
     int pad2_;
 };
 

@@ -72,6 +72,7 @@ lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
 lib.vfd_diffuse_update.argtypes = [_p, ctypes.POINTER(Field), ctypes.POINTER(_p), ctypes.POINTER(ctypes.c_float),
                                    _p, _p, ctypes.POINTER(_p), ctypes.POINTER(_d), _p]
+lib.vfd_diffuse_sample.argtypes = [_p, ctypes.POINTER(Field), _i, ctypes.POINTER(_i), _i, ctypes.POINTER(_p), _p]
 lib.vfd_sample_chem.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
 lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
                                 ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
@@ -183,6 +184,17 @@ class Plan:
         check(lib.vfd_diffuse_update(self.h, arr, dv, dec, _ptr(frag_col), _ptr(frag_ela), gv,
                                      out if totals else None, _stream(stream)))
         return list(out) if totals else None
+
+    def diffuse_sample(self, fields, chem_index, stride, vis, stream=None):
+        """diffuse() with the visualisation downsample of each field's output
+        (sample_chem with chem_index[i]) written into vis[i]."""
+        arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
+        n = len(arr)
+        if len(chem_index) != n or len(vis) != n:
+            raise ValueError("diffuse_sample: one chem index and one output array per field")
+        ci = (ctypes.c_int * max(1, n))(*[int(c) for c in chem_index])
+        vv = (ctypes.c_void_p * max(1, n))(*[x.data_ptr() for x in vis])
+        check(lib.vfd_diffuse_sample(self.h, arr, n, ci, int(stride), vv, _stream(stream)))
 
     def pass_bytes(self, pass_index):
         return lib.vfd_plan_pass_bytes(self.h, int(pass_index))
