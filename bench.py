@@ -62,8 +62,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="128,512,1024", help="nz,ny,nx of the data box")
     ap.add_argument("--box", default=None, help="dropin: FFT box FD,FH,FW (default twice the data box)")
-    ap.add_argument("--flows", default="pipelined,serial,resident",
-                    help="dropin: flows to time (the first one named is the reported value)")
+    ap.add_argument("--flows", default="pipelined,serial,resident,tick",
+                    help="dropin: flows to time (the first one named is the reported value); tick = "
+                         "vfdTickChems, the diffusion and the chem update with resident levels")
     ap.add_argument("--fields", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -387,6 +388,24 @@ def bench_dropin(args):
             host[ic].copy_(dev[ic])
             torch.cuda.synchronize()
 
+    # the resident tick (vfdTickChems): levels stay in d_data, deltas in, levels
+    # and gradients out, the chem update fused into the diffusion
+    tick_state = {}
+
+    def tick():
+        if not tick_state:
+            tick_state["d"] = [torch.zeros((nz, ny, nx), dtype=torch.float32).pin_memory() for _ in range(nf)]
+            tick_state["frag"] = [torch.rand((nz, ny, nx), dtype=torch.float32).pin_memory() for _ in range(2)]
+            tick_state["g"] = [torch.empty((nz, ny, nx), dtype=torch.float32).pin_memory() for _ in range(3)]
+            tick_state["dv"] = (ctypes.c_void_p * nf)(*[x.data_ptr() for x in tick_state["d"]])
+            tick_state["gv"] = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in tick_state["g"]])
+            tick_state["decay"] = (ctypes.c_float * nf)(0.319, 0.00003052, 0.67, 0.2, 0.2, 0.7937, 0.9284, 0.2)
+            tick_state["tot"] = (ctypes.c_double * nf)()
+        st = tick_state
+        if not legacy.lib.vfdTickChems(cc, epi, base, st["dv"], st["decay"], st["frag"][0].data_ptr(),
+                                       st["frag"][1].data_ptr(), st["gv"], st["tot"]):
+            raise vfd.VfdError(-1, vfd.last_error())
+
     def resident():
         for ic in range(nf):
             if not legacy.lib.fftDiffuse3D(dev[ic].data_ptr(), spec[ic].data_ptr(), 0, 0, cc, epi,
@@ -405,7 +424,8 @@ def bench_dropin(args):
 
     K = max(1, args.steps // 10)
     flows = {"pipelined": ("vfdDiffuseChems", pipelined), "serial": ("serial_h2d_fftDiffuse3D_d2h", serial),
-             "resident": ("device_resident_fftDiffuse3D", resident)}
+             "resident": ("device_resident_fftDiffuse3D", resident),
+             "tick": ("vfdTickChems_diffusion_and_update", tick)}
     names = [f.strip() for f in args.flows.split(",") if f.strip()]
     if not names or any(n not in flows for n in names):
         raise SystemExit(f"--flows: choose from {','.join(flows)}")

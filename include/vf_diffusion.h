@@ -139,6 +139,23 @@ bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlan
  * Synchronous on return; false (and vfd_last_error) on bad arguments. ---- */
 bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const float* baseChem);
 
+/* ---- one ABM tick of the 8 chems with the levels resident on the device:
+ * Diffuser::diffuseChemGPU (Diffuser.cpp:221-470) followed by
+ * WHChemical::update with gpuUpdate (WHChemical.cpp:395-450) in one call, with
+ * only the agents' deltas crossing PCIe towards the device.  d_data[ic] holds
+ * chem ic's levels (pChem) from the previous tick and keeps the new ones; the
+ * caller loads it once (H2D) before the first tick, one buffer per chem.
+ * Per tick: h_dchem[ic] -> device; the fftDiffuse3D operator on d_data[ic]
+ * with p = d + t * decay[ic] fused into its last pass (FP32 product rounded
+ * before the add) and the device delta cleared; new levels -> h_ibuffs[ic]
+ * (pChem); h_dchem[ic] zeroed on the host; the ECM fragment fields
+ * (fragmnt_col, fragmnt_ela of WHWorld::worldECM, packed) -> device; the
+ * NEU, MAC, FIB gradients -> h_grads[0..2]; compensated totals -> totals[8].
+ * Bit-identical to vfdDiffuseChems followed by the host update.  Synchronous
+ * on return; false (and vfd_last_error) on bad arguments. ---- */
+bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem, float* const* h_dchem, const float* decay,
+                  const float* h_frag_col, const float* h_frag_ela, float* const* h_grads, double* totals);
+
 /* ---- memory for the diffusion_helper / Diffuser layer and WHChemical: stands
  * in for the cudaGetDeviceCount / cudaSetDevice / cudaMalloc / cudaFree /
  * cudaMallocHost / cudaFreeHost calls of DiffusionHelper.cpp:6-26, 160-225,

@@ -313,3 +313,94 @@ def test_fftDiffuse3D_production_shape(vfd, orc, leg):
         got = ud[lo[0]:lo[0] + n[0], lo[1]:lo[1] + n[1], lo[2]:lo[2] + n[2]].double().cpu().numpy()
         print(f"  sub-box at {lo}:", end="", flush=True)
         assert _rel(got, ref) <= 1e-5
+
+
+def test_vfdTickChems_matches_diffuse_then_update(vfd, orc, leg):
+    """The resident tick (vfdTickChems: deltas in, diffusion with the update
+    fused, levels and gradients out) against the reference flow: vfdDiffuseChems
+    (pChem -> tChem) then the host update of the oracle.  Levels, gradients and
+    cleared deltas bit-exact, totals to 1e-12; two ticks, so the second starts
+    from the levels kept on the device."""
+    import torch
+    shape, box, epi = (10, 12, 14), (24, 24, 32), 12
+    print(f"case vfdTickChems shape={shape} box={box}", flush=True)
+    rng = np.random.default_rng(21)
+    n = int(np.prod(shape))
+    K = 5
+    specs = []
+    for ic in range(8):
+        w = rng.random((K, K, K))
+        w = (w + w[::-1, ::-1, ::-1]) / (2 * w.sum())
+        specs.append(_t(orc.kernel_spectrum(w, (2, 2, 2), box).astype(np.complex64)))
+    base = (ctypes.c_float * 8)(*[0.01 * (ic + 1) for ic in range(8)])
+    decay = [0.319, 0.00003052, 0.67, 0.2, 0.2, 0.7937, 0.9284, 0.2]   # woundHealingWorld.cpp:2636-2643
+    dec = (ctypes.c_float * 8)(*decay)
+    pin = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).pin_memory()
+
+    def cctx(d_data, h_in, h_out):
+        cc = leg.CCtx()
+        cc.DD, cc.DH, cc.DW = shape
+        cc.FFTD, cc.FFTH, cc.FFTW = box
+        cc.KD = cc.KH = cc.KW = K
+        cc.KZ = cc.KY = cc.KX = 2
+        for ic in range(8):
+            cc.d_data[ic] = d_data[ic].data_ptr()
+            cc.h_ibuffs[ic] = h_in[ic].data_ptr()
+            cc.h_obuffs[ic] = h_out[ic].data_ptr() if h_out else 0
+            cc.d_kernelspectrum_h[ic] = specs[ic].data_ptr()
+            cc.dev_id[ic] = 0
+        return cc
+
+    levels = [rng.lognormal(-2, 1, shape).astype(np.float32) for _ in range(8)]
+    dev_levels = [_t(x) for x in levels]
+    h_levels = [pin(np.zeros(shape)) for _ in range(8)]
+    cc_tick = cctx(dev_levels, h_levels, None)
+    for tick in range(2):
+        deltas = [(rng.standard_normal(shape) * 0.01).astype(np.float32) for _ in range(8)]
+        fc, fe = rng.random(shape).astype(np.float32), rng.random(shape).astype(np.float32)
+        # reference flow
+        h_p = [pin(x) for x in levels]
+        h_t = [pin(np.zeros(shape)) for _ in range(8)]
+        scratch = [torch.empty(shape, dtype=torch.float32, device="cuda") for _ in range(8)]
+        assert leg.lib.vfdDiffuseChems(cctx(scratch, h_p, h_t), 8, epi, base), vfd.last_error()
+        p_ref, d_ref, g_ref, tot_ref = orc.chem_update([x.numpy().reshape(-1) for x in h_t],
+                                                       [x.reshape(-1) for x in deltas], decay,
+                                                       fc.reshape(-1), fe.reshape(-1))
+        # resident tick
+        h_d = [pin(x) for x in deltas]
+        h_fc, h_fe = pin(fc), pin(fe)
+        h_g = [pin(np.zeros(shape)) for _ in range(3)]
+        tot = (ctypes.c_double * 8)()
+        dv = (ctypes.c_void_p * 8)(*[x.data_ptr() for x in h_d])
+        gv = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in h_g])
+        assert leg.lib.vfdTickChems(cc_tick, epi, base, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
+            vfd.last_error()
+        for ic in range(8):
+            got = h_levels[ic].numpy().reshape(-1)
+            assert np.array_equal(got, p_ref[ic]), (tick, ic)
+            assert np.array_equal(dev_levels[ic].cpu().numpy().reshape(-1), p_ref[ic]), (tick, ic)
+            assert not h_d[ic].numpy().any(), (tick, ic)
+            assert abs(tot[ic] - tot_ref[ic]) <= 1e-12 * abs(tot_ref[ic]), (tick, ic)
+        for g in range(3):
+            assert np.array_equal(h_g[g].numpy().reshape(-1), g_ref[g]), (tick, g)
+        levels = [x.reshape(shape) for x in p_ref]
+
+
+def test_vfdTickChems_rejects_shared_levels(vfd, leg):
+    import torch
+    shape, box = (4, 4, 4), (8, 8, 8)
+    cc = leg.CCtx()
+    cc.DD, cc.DH, cc.DW = shape
+    cc.FFTD, cc.FFTH, cc.FFTW = box
+    d = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    h = torch.zeros(shape, dtype=torch.float32).pin_memory()
+    k = torch.zeros((8, 8, 5), dtype=torch.complex64, device="cuda")
+    for ic in range(8):
+        cc.d_data[ic] = d.data_ptr()           # one buffer shared by every chem: refused
+        cc.h_ibuffs[ic] = h.data_ptr()
+        cc.d_kernelspectrum_h[ic] = k.data_ptr()
+    f8 = (ctypes.c_float * 8)(*([0.5] * 8))
+    dv = (ctypes.c_void_p * 8)(*([h.data_ptr()] * 8))
+    gv = (ctypes.c_void_p * 3)(*([h.data_ptr()] * 3))
+    assert not leg.lib.vfdTickChems(cc, 4, f8, dv, f8, h.data_ptr(), h.data_ptr(), gv, (ctypes.c_double * 8)())
+    assert "share d_data" in vfd.last_error()

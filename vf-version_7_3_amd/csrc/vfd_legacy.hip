@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -432,6 +433,7 @@ namespace {
 struct Pipe {
     hipStream_t in = nullptr, comp = nullptr, out = nullptr;
     hipEvent_t e_in[N_CHEM] = {}, e_comp[N_CHEM] = {}, e_out[N_CHEM] = {};
+    hipEvent_t e_aux[2] = {};
 };
 std::map<int, Pipe> g_pipes;
 
@@ -447,6 +449,8 @@ int pipe_for(int dev, Pipe** out) {
     for (int i = 0; i < N_CHEM; ++i)
         for (hipEvent_t* e : {&p.e_in[i], &p.e_comp[i], &p.e_out[i]})
             if (int rc = hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "chem pipeline event")) return rc;
+    for (hipEvent_t& e : p.e_aux)
+        if (int rc = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "chem pipeline event")) return rc;
     *out = &g_pipes.emplace(dev, p).first->second;
     return VFD_OK;
 }
@@ -490,6 +494,116 @@ extern "C" bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const fl
         if (!rc) rc = hip_check(hipMemcpyAsync(cctx.h_obuffs[ic], d, bytes, hipMemcpyDeviceToHost, pp->out), "D2H");
         if (!rc) rc = hip_check(hipEventRecord(pp->e_out[ic], pp->out), "pipeline event");
     }
+    for (hipStream_t s : {pp->in, pp->comp, pp->out}) {
+        const int src = hip_check(hipStreamSynchronize(s), "pipeline sync");
+        if (!rc) rc = src;
+    }
+    return rc ? fail(rc) : true;
+}
+
+namespace {
+// device arrays of the resident tick, per workspace: the uploaded deltas, the
+// two ECM fragment fields and the three gradients
+struct TickBufs {
+    float* delta[N_CHEM] = {};
+    float* frag[2] = {};
+    float* grad[3] = {};
+};
+std::map<WsKey, TickBufs> g_tick;
+
+int tick_bufs(const WsKey& key, size_t bytes, TickBufs** out) {
+    auto it = g_tick.find(key);
+    if (it == g_tick.end()) {
+        TickBufs b;
+        std::vector<float**> all;
+        for (float*& p : b.delta) all.push_back(&p);
+        for (float*& p : b.frag) all.push_back(&p);
+        for (float*& p : b.grad) all.push_back(&p);
+        int rc = VFD_OK;
+        for (float** p : all)
+            if (!rc) rc = hip_check(hipMalloc((void**)p, bytes), "tick buffers");
+        if (rc) {
+            for (float** p : all)
+                if (*p) (void)hipFree(*p);
+            return rc;
+        }
+        it = g_tick.emplace(key, b).first;
+    }
+    *out = &it->second;
+    return VFD_OK;
+}
+}  // namespace
+
+extern "C" bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem, float* const* h_dchem,
+                             const float* decay, const float* h_frag_col, const float* h_frag_ela,
+                             float* const* h_grads, double* totals) {
+    if (!baseChem || !h_dchem || !decay || !h_frag_col || !h_frag_ela || !h_grads)
+        return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: null argument"));
+    if (cctx.DD < 1 || cctx.DH < 1 || cctx.DW < 1)
+        return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: empty data box"));
+    for (int ic = 0; ic < N_CHEM; ++ic) {
+        if (!cctx.d_data[ic] || !cctx.d_kernelspectrum_h[ic] || !cctx.h_ibuffs[ic] || !h_dchem[ic])
+            return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: chem %d has a null buffer", ic));
+        for (int jc = 0; jc < ic; ++jc)
+            if (cctx.d_data[jc] == cctx.d_data[ic])
+                return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: chems %d and %d share d_data (levels are resident, "
+                                      "one buffer per chem)", jc, ic));
+    }
+    for (int g = 0; g < 3; ++g)
+        if (!h_grads[g]) return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: gradient %d is null", g));
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+    Ws* ws = nullptr;
+    if (int rc = ws_for(dev, cctx, &ws)) return fail(rc);
+    Pipe* pp = nullptr;
+    if (int rc = pipe_for(dev, &pp)) return fail(rc);
+    const long n = (long)cctx.DD * cctx.DH * cctx.DW;
+    const size_t bytes = (size_t)n * sizeof(float);
+    TickBufs* tb = nullptr;
+    if (int rc = tick_bufs(WsKey{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW}, bytes, &tb))
+        return fail(rc);
+    int rc = VFD_OK;
+    for (int ic = 0; ic < N_CHEM && !rc; ++ic) {
+        float* d = cctx.d_data[ic];
+        rc = hip_check(hipMemcpyAsync(tb->delta[ic], h_dchem[ic], bytes, hipMemcpyHostToDevice, pp->in), "H2D delta");
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_in[ic], pp->in), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_in[ic], 0), "pipeline wait");
+        if (!rc) {
+            vfd_field f{d, d, ws->work, cctx.d_kernelspectrum_h[ic], (double)baseChem[ic], epiBoundary, 0};
+            Epilogue ep;
+            ep.deltas = &tb->delta[ic];
+            ep.decay = &decay[ic];
+            rc = plan_diffuse_passes(ws->plan, &f, 1, 0, 5, pp->comp, &ep);
+        }
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_comp[ic], pp->comp), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_comp[ic], 0), "pipeline wait");
+        if (!rc) rc = hip_check(hipMemcpyAsync(cctx.h_ibuffs[ic], d, bytes, hipMemcpyDeviceToHost, pp->out), "D2H levels");
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_out[ic], pp->out), "pipeline event");
+    }
+    if (!rc) rc = hip_check(hipMemcpyAsync(tb->frag[0], h_frag_col, bytes, hipMemcpyHostToDevice, pp->in), "H2D frag");
+    if (!rc) rc = hip_check(hipMemcpyAsync(tb->frag[1], h_frag_ela, bytes, hipMemcpyHostToDevice, pp->in), "H2D frag");
+    if (!rc) rc = hip_check(hipEventRecord(pp->e_aux[0], pp->in), "pipeline event");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_aux[0], 0), "pipeline wait");
+    // the host deltas are cleared once their copy has landed, while the device works
+    std::vector<std::thread> clear;
+    if (!rc)
+        for (int ic = 0; ic < N_CHEM; ++ic)
+            clear.emplace_back([pp, ic, h_dchem, bytes] {
+                (void)hipEventSynchronize(pp->e_in[ic]);
+                std::memset(h_dchem[ic], 0, bytes);
+            });
+    if (!rc) {
+        float* levels[N_CHEM];
+        for (int ic = 0; ic < N_CHEM; ++ic) levels[ic] = cctx.d_data[ic];
+        rc = chem_grads_totals(levels, tb->frag[0], tb->frag[1], tb->grad, n, totals, pp->comp);
+    }
+    if (!rc) rc = hip_check(hipEventRecord(pp->e_aux[1], pp->comp), "pipeline event");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_aux[1], 0), "pipeline wait");
+    for (int g = 0; g < 3 && !rc; ++g)
+        rc = hip_check(hipMemcpyAsync(h_grads[g], tb->grad[g], bytes, hipMemcpyDeviceToHost, pp->out), "D2H gradient");
+    for (std::thread& t : clear) t.join();
     for (hipStream_t s : {pp->in, pp->comp, pp->out}) {
         const int src = hip_check(hipStreamSynchronize(s), "pipeline sync");
         if (!rc) rc = src;

@@ -39,6 +39,10 @@ lib.D2H.argtypes = [_i, ctypes.POINTER(CCtx), _i]
 lib.D2H.restype = None
 lib.vfdDiffuseChems.argtypes = [CCtx, _i, _i, ctypes.POINTER(_f)]
 lib.vfdDiffuseChems.restype = ctypes.c_bool
+lib.vfdTickChems.argtypes = [CCtx, _i, ctypes.POINTER(_f), ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_f),
+                             ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                             ctypes.POINTER(ctypes.c_double)]
+lib.vfdTickChems.restype = ctypes.c_bool
 lib.vfdSetKernelPowerQuirk.argtypes = [_i]
 lib.vfdSetKernelPowerQuirk.restype = None
 for name, n in (("extractCenter3D", 8), ("unpadResult3D", 8), ("padKernel3D", 11),
