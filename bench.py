@@ -62,9 +62,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="128,512,1024", help="nz,ny,nx of the data box")
     ap.add_argument("--box", default=None, help="dropin: FFT box FD,FH,FW (default twice the data box)")
-    ap.add_argument("--flows", default="pipelined,serial,resident,tick",
+    ap.add_argument("--flows", default="pipelined,serial,resident,tick,split",
                     help="dropin: flows to time (the first one named is the reported value); tick = "
-                         "vfdTickChems, the diffusion and the chem update with resident levels")
+                         "vfdTickChems, the diffusion and the chem update with resident levels; split = "
+                         "vfdTickChemsEnd alone, the diffusion having run during the agents' step")
     ap.add_argument("--fields", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -406,6 +407,22 @@ def bench_dropin(args):
                                        st["frag"][1].data_ptr(), st["gv"], st["tot"]):
             raise vfd.VfdError(-1, vfd.last_error())
 
+    def split_end():
+        # vfdTickChemsBegin, the device left to finish the diffusion (the agents'
+        # step on the host), then vfdTickChemsEnd alone is timed
+        tick()  # allocates the state on first use
+        st = tick_state
+        if not legacy.lib.vfdTickChemsBegin(cc, epi, base):
+            raise vfd.VfdError(-1, vfd.last_error())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if not legacy.lib.vfdTickChemsEnd(cc, st["dv"], st["decay"], st["frag"][0].data_ptr(),
+                                          st["frag"][1].data_ptr(), st["gv"], st["tot"]):
+            raise vfd.VfdError(-1, vfd.last_error())
+        split_times.append(time.perf_counter() - t0)
+
+    split_times = []
+
     def resident():
         for ic in range(nf):
             if not legacy.lib.fftDiffuse3D(dev[ic].data_ptr(), spec[ic].data_ptr(), 0, 0, cc, epi,
@@ -425,11 +442,14 @@ def bench_dropin(args):
     K = max(1, args.steps // 10)
     flows = {"pipelined": ("vfdDiffuseChems", pipelined), "serial": ("serial_h2d_fftDiffuse3D_d2h", serial),
              "resident": ("device_resident_fftDiffuse3D", resident),
-             "tick": ("vfdTickChems_diffusion_and_update", tick)}
+             "tick": ("vfdTickChems_diffusion_and_update", tick),
+             "split": ("vfdTickChemsEnd_after_agents", split_end)}
     names = [f.strip() for f in args.flows.split(",") if f.strip()]
     if not names or any(n not in flows for n in names):
         raise SystemExit(f"--flows: choose from {','.join(flows)}")
     t = {n: timed(flows[n][1], K) for n in names}
+    if "split" in t:  # only the End call of each timed split tick
+        t["split"] = sum(split_times[-K:]) / K
     vox = float(nx * ny * nz) * nf
     first = t[names[0]]
     metric = ("voxel-updates/s of the drop-in tick through the reference C interface"

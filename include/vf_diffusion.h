@@ -156,6 +156,20 @@ bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const float* baseCh
 bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem, float* const* h_dchem, const float* decay,
                   const float* h_frag_col, const float* h_frag_ela, float* const* h_grads, double* totals);
 
+/* ---- vfdTickChems split at the agents' step, the go() split of
+ * woundHealingWorld.cpp:2013-2055 (diffusion on one OpenMP thread while the
+ * agents run on another): vfdTickChemsBegin queues the diffusion of every
+ * chem's resident levels d_data[ic] (in place: they become tChem) and returns
+ * without waiting, so the device diffuses while the agents write dChem and
+ * read pChem on the host; vfdTickChemsEnd, after the agents, does the rest of
+ * vfdTickChems (deltas and fragments in, p = d + t * decay, gradients and
+ * totals, levels and gradients out, host dChem cleared) and returns when all
+ * is done.  The numbers equal vfdTickChems'.  Between the two calls no other
+ * diffusion entry point of this interface runs (they return false). ---- */
+bool vfdTickChemsBegin(c_ctx cctx, int epiBoundary, const float* baseChem);
+bool vfdTickChemsEnd(c_ctx cctx, float* const* h_dchem, const float* decay, const float* h_frag_col,
+                     const float* h_frag_ela, float* const* h_grads, double* totals);
+
 /* ---- memory for the diffusion_helper / Diffuser layer and WHChemical: stands
  * in for the cudaGetDeviceCount / cudaSetDevice / cudaMalloc / cudaFree /
  * cudaMallocHost / cudaFreeHost calls of DiffusionHelper.cpp:6-26, 160-225,

@@ -136,3 +136,35 @@ def test_legacy_bad_arguments_do_not_exit(lib):
     from vfd import legacy  # noqa: F401  (argtypes)
     cc = legacy.CCtx()
     assert not legacy.lib.fftDiffuse3D(None, None, 0, 0, cc, 1, 0.0, 0, 0)
+
+
+def test_resident_tick_argument_checks_before_device(lib):
+    """vfdTickChems / vfdTickChemsBegin / vfdTickChemsEnd validate on the host
+    (no device call): null arguments, levels shared between chems, and an End
+    without a pending Begin are errors with a message, not exits."""
+    from vfd import legacy
+    import vfd
+    cc = legacy.CCtx()
+    cc.DD, cc.DH, cc.DW = 4, 4, 4
+    cc.FFTD, cc.FFTH, cc.FFTW = 8, 8, 8
+    f8 = (ctypes.c_float * 8)(*([0.5] * 8))
+    buf = ctypes.create_string_buffer(4 * 64)
+    p = ctypes.cast(buf, ctypes.c_void_p).value
+    dv = (ctypes.c_void_p * 8)(*([p] * 8))
+    gv = (ctypes.c_void_p * 3)(*([p] * 3))
+    tot = (ctypes.c_double * 8)()
+    assert not legacy.lib.vfdTickChems(cc, 4, None, dv, f8, p, p, gv, tot)
+    assert "null argument" in vfd.last_error()
+    for ic in range(8):
+        cc.d_data[ic] = p                      # one buffer for every chem
+        cc.h_ibuffs[ic] = p
+        cc.d_kernelspectrum_h[ic] = p
+    assert not legacy.lib.vfdTickChems(cc, 4, f8, dv, f8, p, p, gv, tot)
+    assert "share d_data" in vfd.last_error()
+    assert not legacy.lib.vfdTickChemsBegin(cc, 4, f8)
+    assert "share d_data" in vfd.last_error()
+    bufs = [ctypes.create_string_buffer(4 * 64) for _ in range(8)]
+    for ic in range(8):
+        cc.d_data[ic] = ctypes.cast(bufs[ic], ctypes.c_void_p).value
+    assert not legacy.lib.vfdTickChemsEnd(cc, dv, f8, p, p, gv, tot)
+    assert "no vfdTickChemsBegin is pending" in vfd.last_error()

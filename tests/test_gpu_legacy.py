@@ -315,15 +315,17 @@ def test_fftDiffuse3D_production_shape(vfd, orc, leg):
         assert _rel(got, ref) <= 1e-5
 
 
-def test_vfdTickChems_matches_diffuse_then_update(vfd, orc, leg):
+@pytest.mark.parametrize("split", [False, True])
+def test_vfdTickChems_matches_diffuse_then_update(vfd, orc, leg, split):
     """The resident tick (vfdTickChems: deltas in, diffusion with the update
-    fused, levels and gradients out) against the reference flow: vfdDiffuseChems
-    (pChem -> tChem) then the host update of the oracle.  Levels, gradients and
-    cleared deltas bit-exact, totals to 1e-12; two ticks, so the second starts
-    from the levels kept on the device."""
+    fused, levels and gradients out; or its split form vfdTickChemsBegin /
+    vfdTickChemsEnd around the agents' step) against the reference flow:
+    vfdDiffuseChems (pChem -> tChem) then the host update of the oracle.
+    Levels, gradients and cleared deltas bit-exact, totals to 1e-12; two ticks,
+    so the second starts from the levels kept on the device."""
     import torch
     shape, box, epi = (10, 12, 14), (24, 24, 32), 12
-    print(f"case vfdTickChems shape={shape} box={box}", flush=True)
+    print(f"case vfdTickChems split={split} shape={shape} box={box}", flush=True)
     rng = np.random.default_rng(21)
     n = int(np.prod(shape))
     K = 5
@@ -373,8 +375,16 @@ def test_vfdTickChems_matches_diffuse_then_update(vfd, orc, leg):
         tot = (ctypes.c_double * 8)()
         dv = (ctypes.c_void_p * 8)(*[x.data_ptr() for x in h_d])
         gv = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in h_g])
-        assert leg.lib.vfdTickChems(cc_tick, epi, base, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
-            vfd.last_error()
+        if split:
+            assert leg.lib.vfdTickChemsBegin(cc_tick, epi, base), vfd.last_error()
+            # no other diffusion while the split tick is pending
+            assert not leg.lib.vfdDiffuseChems(cctx(scratch, h_p, h_t), 8, epi, base)
+            assert "pending" in vfd.last_error()
+            assert leg.lib.vfdTickChemsEnd(cc_tick, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
+                vfd.last_error()
+        else:
+            assert leg.lib.vfdTickChems(cc_tick, epi, base, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
+                vfd.last_error()
         for ic in range(8):
             got = h_levels[ic].numpy().reshape(-1)
             assert np.array_equal(got, p_ref[ic]), (tick, ic)

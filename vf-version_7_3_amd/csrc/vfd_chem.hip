@@ -148,6 +148,19 @@ __global__ void sample_chem_kernel(const float* __restrict__ src, int H, int W, 
     }
 }
 
+// one chem's levels: p = d + t * decay (product rounded before the add), d = 0;
+// p may alias t
+__global__ void __launch_bounds__(kThreads)
+chem_levels_kernel(float* p, const float* t, float* __restrict__ d, float decay, long n) {
+#pragma clang fp contract(off)
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float scaled = t[i] * decay;
+        const float lv = d[i] + scaled;
+        d[i] = 0.0f;
+        p[i] = lv;
+    }
+}
+
 }  // namespace
 
 extern "C" int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem_index, int stride, float* d_dst,
@@ -197,6 +210,14 @@ static int chem_pass(const ChemPtrs& cp, bool levels_in_p, const float* d_frag_c
 }
 
 namespace vfd {
+// one chem's level update (the first half of vfd_chem_update for that chem)
+int chem_levels(float* d_p, const float* d_t, float* d_d, float decay, long n, hipStream_t st) {
+    if (n <= 0) return VFD_OK;
+    const unsigned blocks = (unsigned)std::min<long>(kBlocks * 4, (n + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(chem_levels_kernel, dim3(blocks), dim3(kThreads), 0, st, d_p, d_t, d_d, decay, n);
+    return hip_check(hipGetLastError(), "chem levels launch");
+}
+
 // display scales per chem (volumeRender_kernel.cu:258-294)
 float chem_display_scale(int chem) {
     static const float scale[kChems] = {50000.0f, 10000.0f, 1000000.0f, 100.0f, 10000.0f, 100000.0f, 100000.0f, 10000.0f};

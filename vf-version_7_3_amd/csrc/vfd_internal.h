@@ -32,6 +32,7 @@ void tuned_lookup(Plan& q);
 int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms);
 int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
 float chem_display_scale(int chem);  // sampleChem's per-chem display scale, chem < 8
+int chem_levels(float* d_p, const float* d_t, float* d_d, float decay, long n, hipStream_t st);
 int chem_grads_totals(float* const* d_p, const float* d_frag_col, const float* d_frag_ela, float* const* d_grads,
                       long n, double* totals, hipStream_t st);
 // Optional work of the x-inverse epilogue (FP32 plans), per field f:

@@ -231,6 +231,14 @@ __global__ void pad_mirror2d(float* dst, const float* src, int fH, int fW, int H
 
 // ------------------------------------------------------------------ state
 std::mutex g_mu;  // serializes fftDiffuse3D / kernel precompute (shared workspace)
+// a vfdTickChemsBegin whose diffusions may still run on the shared workspace
+// (guarded by g_mu); the other diffusion entry points refuse to run meanwhile
+bool g_split_pending = false;
+int refuse_if_split_pending(const char* who) {
+    return g_split_pending ? set_error(VFD_ERR_INVALID, "%s: a vfdTickChemsBegin is pending; call vfdTickChemsEnd first",
+                                       who)
+                           : VFD_OK;
+}
 int g_quirk = 1;
 
 struct PlanRec {
@@ -404,6 +412,7 @@ bool fftDiffuse3D(float* d_Data, fComplex* d_KernelSpectrum, cufftHandle fftPlan
     (void)iter;
     if (!d_Data || !d_KernelSpectrum) return fail(set_error(VFD_ERR_INVALID, "fftDiffuse3D: null buffer"));
     std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = refuse_if_split_pending("fftDiffuse3D")) return fail(rc);
     for (cufftHandle h : {fftPlanFwd, fftPlanInv}) {
         if (h == 0) continue;  // 0: no plan supplied, the box comes from cctx
         auto it = g_plans.find(h);
@@ -466,6 +475,7 @@ extern "C" bool vfdDiffuseChems(c_ctx cctx, int nchem, int epiBoundary, const fl
             return fail(set_error(VFD_ERR_INVALID, "vfdDiffuseChems: chem %d has a null buffer", ic));
     if (nchem == 0) return true;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = refuse_if_split_pending("vfdDiffuseChems")) return fail(rc);
     if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
@@ -534,24 +544,73 @@ int tick_bufs(const WsKey& key, size_t bytes, TickBufs** out) {
 }
 }  // namespace
 
+namespace {
+// arguments of the resident tick: the diffusion half (base) and the update
+// half (deltas, fragments, gradients); a null group is not checked
+int tick_check(const char* who, const c_ctx& cctx, const float* baseChem, bool diffuse, float* const* h_dchem,
+               const float* decay, const float* h_frag_col, const float* h_frag_ela, float* const* h_grads,
+               bool update) {
+    if ((diffuse && !baseChem) || (update && (!h_dchem || !decay || !h_frag_col || !h_frag_ela || !h_grads)))
+        return set_error(VFD_ERR_INVALID, "%s: null argument", who);
+    if (cctx.DD < 1 || cctx.DH < 1 || cctx.DW < 1) return set_error(VFD_ERR_INVALID, "%s: empty data box", who);
+    for (int ic = 0; ic < N_CHEM; ++ic) {
+        if (!cctx.d_data[ic] || (diffuse && !cctx.d_kernelspectrum_h[ic]) ||
+            (update && (!cctx.h_ibuffs[ic] || !h_dchem[ic])))
+            return set_error(VFD_ERR_INVALID, "%s: chem %d has a null buffer", who, ic);
+        for (int jc = 0; jc < ic; ++jc)
+            if (cctx.d_data[jc] == cctx.d_data[ic])
+                return set_error(VFD_ERR_INVALID,
+                                 "%s: chems %d and %d share d_data (levels are resident, one buffer per chem)", who, jc,
+                                 ic);
+    }
+    if (update)
+        for (int g = 0; g < 3; ++g)
+            if (!h_grads[g]) return set_error(VFD_ERR_INVALID, "%s: gradient %d is null", who, g);
+    return VFD_OK;
+}
+
+// the host deltas are cleared once their copy has landed, while the device works
+std::vector<std::thread> clear_host_deltas(Pipe* pp, float* const* h_dchem, size_t bytes) {
+    std::vector<std::thread> clear;
+    for (int ic = 0; ic < N_CHEM; ++ic)
+        clear.emplace_back([pp, ic, h_dchem, bytes] {
+            (void)hipEventSynchronize(pp->e_in[ic]);
+            std::memset(h_dchem[ic], 0, bytes);
+        });
+    return clear;
+}
+
+// levels d_data -> h_ibuffs and the three gradients -> h_grads on the out
+// stream, after the compute stream's work so far
+int download_levels_grads(const c_ctx& cctx, Pipe* pp, TickBufs* tb, float* const* h_grads, size_t bytes,
+                          bool levels) {
+    int rc = hip_check(hipEventRecord(pp->e_aux[1], pp->comp), "pipeline event");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_aux[1], 0), "pipeline wait");
+    for (int ic = 0; ic < N_CHEM && levels && !rc; ++ic)
+        rc = hip_check(hipMemcpyAsync(cctx.h_ibuffs[ic], cctx.d_data[ic], bytes, hipMemcpyDeviceToHost, pp->out),
+                       "D2H levels");
+    for (int g = 0; g < 3 && !rc; ++g)
+        rc = hip_check(hipMemcpyAsync(h_grads[g], tb->grad[g], bytes, hipMemcpyDeviceToHost, pp->out), "D2H gradient");
+    return rc;
+}
+
+int sync_pipe(Pipe* pp, int rc) {
+    for (hipStream_t s : {pp->in, pp->comp, pp->out}) {
+        const int src = hip_check(hipStreamSynchronize(s), "pipeline sync");
+        if (!rc) rc = src;
+    }
+    return rc;
+}
+}  // namespace
+
 extern "C" bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem, float* const* h_dchem,
                              const float* decay, const float* h_frag_col, const float* h_frag_ela,
                              float* const* h_grads, double* totals) {
-    if (!baseChem || !h_dchem || !decay || !h_frag_col || !h_frag_ela || !h_grads)
-        return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: null argument"));
-    if (cctx.DD < 1 || cctx.DH < 1 || cctx.DW < 1)
-        return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: empty data box"));
-    for (int ic = 0; ic < N_CHEM; ++ic) {
-        if (!cctx.d_data[ic] || !cctx.d_kernelspectrum_h[ic] || !cctx.h_ibuffs[ic] || !h_dchem[ic])
-            return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: chem %d has a null buffer", ic));
-        for (int jc = 0; jc < ic; ++jc)
-            if (cctx.d_data[jc] == cctx.d_data[ic])
-                return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: chems %d and %d share d_data (levels are resident, "
-                                      "one buffer per chem)", jc, ic));
-    }
-    for (int g = 0; g < 3; ++g)
-        if (!h_grads[g]) return fail(set_error(VFD_ERR_INVALID, "vfdTickChems: gradient %d is null", g));
+    if (int rc = tick_check("vfdTickChems", cctx, baseChem, true, h_dchem, decay, h_frag_col, h_frag_ela, h_grads,
+                            true))
+        return fail(rc);
     std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = refuse_if_split_pending("vfdTickChems")) return fail(rc);
     if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
@@ -586,28 +645,95 @@ extern "C" bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem,
     if (!rc) rc = hip_check(hipMemcpyAsync(tb->frag[1], h_frag_ela, bytes, hipMemcpyHostToDevice, pp->in), "H2D frag");
     if (!rc) rc = hip_check(hipEventRecord(pp->e_aux[0], pp->in), "pipeline event");
     if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_aux[0], 0), "pipeline wait");
-    // the host deltas are cleared once their copy has landed, while the device works
     std::vector<std::thread> clear;
-    if (!rc)
-        for (int ic = 0; ic < N_CHEM; ++ic)
-            clear.emplace_back([pp, ic, h_dchem, bytes] {
-                (void)hipEventSynchronize(pp->e_in[ic]);
-                std::memset(h_dchem[ic], 0, bytes);
-            });
+    if (!rc) clear = clear_host_deltas(pp, h_dchem, bytes);
     if (!rc) {
         float* levels[N_CHEM];
         for (int ic = 0; ic < N_CHEM; ++ic) levels[ic] = cctx.d_data[ic];
         rc = chem_grads_totals(levels, tb->frag[0], tb->frag[1], tb->grad, n, totals, pp->comp);
     }
-    if (!rc) rc = hip_check(hipEventRecord(pp->e_aux[1], pp->comp), "pipeline event");
-    if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_aux[1], 0), "pipeline wait");
-    for (int g = 0; g < 3 && !rc; ++g)
-        rc = hip_check(hipMemcpyAsync(h_grads[g], tb->grad[g], bytes, hipMemcpyDeviceToHost, pp->out), "D2H gradient");
+    if (!rc) rc = download_levels_grads(cctx, pp, tb, h_grads, bytes, false);
     for (std::thread& t : clear) t.join();
-    for (hipStream_t s : {pp->in, pp->comp, pp->out}) {
-        const int src = hip_check(hipStreamSynchronize(s), "pipeline sync");
-        if (!rc) rc = src;
+    rc = sync_pipe(pp, rc);
+    return rc ? fail(rc) : true;
+}
+
+// The resident tick split at the agents' step (the go() split of
+// woundHealingWorld.cpp:2013-2055): Begin queues the diffusion of every chem's
+// resident levels in place (t, the reference's tChem) and returns at once, so
+// it runs while the agents work on the host; End, after the agents, brings the
+// deltas and fragments in, forms p = d + t * decay, the gradients and totals,
+// and copies levels and gradients out.  Same numbers as vfdTickChems.
+extern "C" bool vfdTickChemsBegin(c_ctx cctx, int epiBoundary, const float* baseChem) {
+    if (int rc = tick_check("vfdTickChemsBegin", cctx, baseChem, true, nullptr, nullptr, nullptr, nullptr, nullptr,
+                            false))
+        return fail(rc);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = refuse_if_split_pending("vfdTickChemsBegin")) return fail(rc);
+    if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+    Ws* ws = nullptr;
+    if (int rc = ws_for(dev, cctx, &ws)) return fail(rc);
+    Pipe* pp = nullptr;
+    if (int rc = pipe_for(dev, &pp)) return fail(rc);
+    int rc = VFD_OK;
+    for (int ic = 0; ic < N_CHEM && !rc; ++ic) {
+        vfd_field f{cctx.d_data[ic], cctx.d_data[ic], ws->work, cctx.d_kernelspectrum_h[ic], (double)baseChem[ic],
+                    epiBoundary, 0};
+        rc = plan_diffuse(ws->plan, &f, 1, pp->comp);
     }
+    if (rc) {
+        (void)sync_pipe(pp, rc);
+        return fail(rc);
+    }
+    g_split_pending = true;
+    return true;
+}
+
+extern "C" bool vfdTickChemsEnd(c_ctx cctx, float* const* h_dchem, const float* decay, const float* h_frag_col,
+                                const float* h_frag_ela, float* const* h_grads, double* totals) {
+    if (int rc = tick_check("vfdTickChemsEnd", cctx, nullptr, false, h_dchem, decay, h_frag_col, h_frag_ela, h_grads,
+                            true))
+        return fail(rc);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_split_pending) return fail(set_error(VFD_ERR_INVALID, "vfdTickChemsEnd: no vfdTickChemsBegin is pending"));
+    int dev = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+    Pipe* pp = nullptr;
+    if (int rc = pipe_for(dev, &pp)) return fail(rc);
+    const long n = (long)cctx.DD * cctx.DH * cctx.DW;
+    const size_t bytes = (size_t)n * sizeof(float);
+    TickBufs* tb = nullptr;
+    int rc = tick_bufs(WsKey{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW}, bytes, &tb);
+    // per chem, pipelined: delta in, p = d + t * decay in place over the
+    // diffused levels (d cleared), levels out
+    for (int ic = 0; ic < N_CHEM && !rc; ++ic) {
+        rc = hip_check(hipMemcpyAsync(tb->delta[ic], h_dchem[ic], bytes, hipMemcpyHostToDevice, pp->in), "H2D delta");
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_in[ic], pp->in), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_in[ic], 0), "pipeline wait");
+        if (!rc) rc = chem_levels(cctx.d_data[ic], cctx.d_data[ic], tb->delta[ic], decay[ic], n, pp->comp);
+        if (!rc) rc = hip_check(hipEventRecord(pp->e_comp[ic], pp->comp), "pipeline event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(pp->out, pp->e_comp[ic], 0), "pipeline wait");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(cctx.h_ibuffs[ic], cctx.d_data[ic], bytes, hipMemcpyDeviceToHost, pp->out),
+                           "D2H levels");
+    }
+    if (!rc) rc = hip_check(hipMemcpyAsync(tb->frag[0], h_frag_col, bytes, hipMemcpyHostToDevice, pp->in), "H2D frag");
+    if (!rc) rc = hip_check(hipMemcpyAsync(tb->frag[1], h_frag_ela, bytes, hipMemcpyHostToDevice, pp->in), "H2D frag");
+    if (!rc) rc = hip_check(hipEventRecord(pp->e_aux[0], pp->in), "pipeline event");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(pp->comp, pp->e_aux[0], 0), "pipeline wait");
+    std::vector<std::thread> clear;
+    if (!rc) clear = clear_host_deltas(pp, h_dchem, bytes);
+    if (!rc) {
+        float* lv[N_CHEM];
+        for (int ic = 0; ic < N_CHEM; ++ic) lv[ic] = cctx.d_data[ic];
+        rc = chem_grads_totals(lv, tb->frag[0], tb->frag[1], tb->grad, n, totals, pp->comp);
+    }
+    if (!rc) rc = download_levels_grads(cctx, pp, tb, h_grads, bytes, false);
+    for (std::thread& t : clear) t.join();
+    rc = sync_pipe(pp, rc);
+    g_split_pending = false;
     return rc ? fail(rc) : true;
 }
 

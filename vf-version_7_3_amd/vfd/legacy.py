@@ -43,6 +43,11 @@ lib.vfdTickChems.argtypes = [CCtx, _i, ctypes.POINTER(_f), ctypes.POINTER(ctypes
                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
                              ctypes.POINTER(ctypes.c_double)]
 lib.vfdTickChems.restype = ctypes.c_bool
+lib.vfdTickChemsBegin.argtypes = [CCtx, _i, ctypes.POINTER(_f)]
+lib.vfdTickChemsBegin.restype = ctypes.c_bool
+lib.vfdTickChemsEnd.argtypes = [CCtx, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_f), ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_double)]
+lib.vfdTickChemsEnd.restype = ctypes.c_bool
 lib.vfdSetKernelPowerQuirk.argtypes = [_i]
 lib.vfdSetKernelPowerQuirk.restype = None
 for name, n in (("extractCenter3D", 8), ("unpadResult3D", 8), ("padKernel3D", 11),
