@@ -166,6 +166,13 @@ bool vfdTickChems(c_ctx cctx, int epiBoundary, const float* baseChem, float* con
  * totals, levels and gradients out, host dChem cleared) and returns when all
  * is done.  The numbers equal vfdTickChems'.  Between the two calls no other
  * diffusion entry point of this interface runs (they return false). ---- */
+/* ---- frees what the interface keeps between calls (the per-box workspaces of
+ * fftDiffuse3D / vfdDiffuseChems / vfdTickChems, the resident tick's arrays,
+ * the pipeline's streams and events); later calls make them again.  0, or a
+ * negative vfd error code (a pending vfdTickChemsBegin is one).  Called by
+ * diffusion_helper::deallocConvCtxBuffers. ---- */
+int vfdReleaseWorkspaces(void);
+
 bool vfdTickChemsBegin(c_ctx cctx, int epiBoundary, const float* baseChem);
 bool vfdTickChemsEnd(c_ctx cctx, float* const* h_dchem, const float* decay, const float* h_frag_col,
                      const float* h_frag_ela, float* const* h_grads, double* totals);

@@ -168,3 +168,9 @@ def test_resident_tick_argument_checks_before_device(lib):
         cc.d_data[ic] = ctypes.cast(bufs[ic], ctypes.c_void_p).value
     assert not legacy.lib.vfdTickChemsEnd(cc, dv, f8, p, p, gv, tot)
     assert "no vfdTickChemsBegin is pending" in vfd.last_error()
+
+
+def test_release_workspaces_with_nothing_held(lib):
+    """vfdReleaseWorkspaces with no workspace made is a no-op returning 0 (no device call)."""
+    lib.vfdReleaseWorkspaces.restype = ctypes.c_int
+    assert lib.vfdReleaseWorkspaces() == 0

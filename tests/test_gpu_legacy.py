@@ -414,3 +414,28 @@ def test_vfdTickChems_rejects_shared_levels(vfd, leg):
     gv = (ctypes.c_void_p * 3)(*([h.data_ptr()] * 3))
     assert not leg.lib.vfdTickChems(cc, 4, f8, dv, f8, h.data_ptr(), h.data_ptr(), gv, (ctypes.c_double * 8)())
     assert "share d_data" in vfd.last_error()
+
+
+def test_release_workspaces_then_diffuse_again(vfd, orc, leg):
+    """vfdReleaseWorkspaces frees the per-box workspace; the next fftDiffuse3D
+    makes it again and gives the same result."""
+    import torch
+    shape, box, K, epi, base = (10, 12, 9), (24, 24, 24), 7, 7, 0.5
+    u = orc.synth_field(shape, 5, nsrc=4)
+    rng = np.random.default_rng(11)
+    w = rng.random((K, K, K))
+    w = (w + w[::-1, ::-1, ::-1]) / (2 * w.sum())
+    spec = _t(orc.kernel_spectrum(w, (3, 3, 3), box).astype(np.complex64))
+    cc = leg.CCtx()
+    cc.DD, cc.DH, cc.DW = shape
+    cc.FFTD, cc.FFTH, cc.FFTW = box
+    outs = []
+    for _ in range(2):
+        d = _t(u)
+        assert leg.lib.fftDiffuse3D(ctypes.c_void_p(d.data_ptr()), ctypes.c_void_p(spec.data_ptr()), 0, 0, cc, epi,
+                                    base, 0, 0), vfd.last_error()
+        torch.cuda.synchronize()
+        outs.append(d)
+        leg.lib.vfdReleaseWorkspaces.restype = ctypes.c_int
+        assert leg.lib.vfdReleaseWorkspaces() == 0, vfd.last_error()
+    assert torch.equal(outs[0], outs[1])

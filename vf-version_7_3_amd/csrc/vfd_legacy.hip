@@ -737,6 +737,40 @@ extern "C" bool vfdTickChemsEnd(c_ctx cctx, float* const* h_dchem, const float* 
     return rc ? fail(rc) : true;
 }
 
+// Frees what the interface keeps between calls: the per-box workspaces, the
+// resident-tick arrays and the copy/compute pipeline's streams and events.
+// The next call that needs them makes them again.
+extern "C" int vfdReleaseWorkspaces(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (int rc = refuse_if_split_pending("vfdReleaseWorkspaces")) return rc;
+    int rc = VFD_OK;
+    auto keep = [&rc](int r) {
+        if (!rc) rc = r;
+    };
+    for (auto& kv : g_pipes) {
+        Pipe& p = kv.second;
+        for (hipStream_t s : {p.in, p.comp, p.out}) {
+            keep(hip_check(hipStreamSynchronize(s), "release: pipeline sync"));
+            keep(hip_check(hipStreamDestroy(s), "release: pipeline stream"));
+        }
+        for (int i = 0; i < N_CHEM; ++i)
+            for (hipEvent_t e : {p.e_in[i], p.e_comp[i], p.e_out[i]}) keep(hip_check(hipEventDestroy(e), "release: event"));
+        for (hipEvent_t e : p.e_aux) keep(hip_check(hipEventDestroy(e), "release: event"));
+    }
+    g_pipes.clear();
+    for (auto& kv : g_tick) {
+        TickBufs& b = kv.second;
+        for (float* q : b.delta) keep(hip_check(hipFree(q), "release: tick arrays"));
+        for (float* q : b.frag) keep(hip_check(hipFree(q), "release: tick arrays"));
+        for (float* q : b.grad) keep(hip_check(hipFree(q), "release: tick arrays"));
+    }
+    g_tick.clear();
+    for (auto& kv : g_ws)
+        if (kv.second.work) keep(hip_check(hipFree(kv.second.work), "release: workspace"));
+    g_ws.clear();
+    return rc;
+}
+
 // --------------------------------------------------------- kernel build
 // computeKernel3D (Diffusion.cu:1078-1531), FP64 on the device: the 7-point
 // stencil at the centre of the kernel domain, niter rounds of clamp-to-border

@@ -165,6 +165,8 @@ void deallocConvCtxBuffers(c_ctx* ctx) {
         ctx->d_data[ic] = nullptr;
         ctx->h_data[ic] = nullptr;
     }
+    // the layer's own workspaces (made again if diffusion continues)
+    if (vfdReleaseWorkspaces() < 0) fprintf(stderr, "diffusion_helper: %s\n", vfd_last_error());
 }
 
 // DiffusionHelper.cpp:319-378
