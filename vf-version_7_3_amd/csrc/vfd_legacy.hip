@@ -283,8 +283,7 @@ int ws_for(int dev, const c_ctx& cctx, Ws** out) {
         const char* fixed = getenv("VFD_SCHEDULE");
         const char* tune = getenv("VFD_AUTOTUNE");
         if (!fixed && !(tune && atoi(tune) == 0)) {
-            // on a stream of its own: work on the null stream would tie the
-            // copy/compute pipeline's streams to it
+            // timed on a stream of its own, away from the caller's work
             hipStream_t ts = nullptr;
             int rc = hip_check(hipStreamCreateWithFlags(&ts, hipStreamNonBlocking), "autotune stream");
             if (!rc) rc = plan_autotune(ws.plan, 1, ts, nullptr);
@@ -698,6 +697,8 @@ extern "C" bool vfdTickChemsEnd(c_ctx cctx, float* const* h_dchem, const float* 
         return fail(rc);
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_split_pending) return fail(set_error(VFD_ERR_INVALID, "vfdTickChemsEnd: no vfdTickChemsBegin is pending"));
+    // End may run on another host thread than Begin did
+    if (!hip_ok(hipSetDevice(cctx.dev_id[0]), "hipSetDevice")) return false;
     int dev = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
     Pipe* pp = nullptr;
