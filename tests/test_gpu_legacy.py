@@ -380,8 +380,15 @@ def test_vfdTickChems_matches_diffuse_then_update(vfd, orc, leg, split):
             # no other diffusion while the split tick is pending
             assert not leg.lib.vfdDiffuseChems(cctx(scratch, h_p, h_t), 8, epi, base)
             assert "pending" in vfd.last_error()
-            assert leg.lib.vfdTickChemsEnd(cc_tick, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
-                vfd.last_error()
+            # End on another host thread, as after the agents' join
+            import threading
+            res = []
+            th = threading.Thread(target=lambda: res.append(
+                (leg.lib.vfdTickChemsEnd(cc_tick, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot),
+                 vfd.last_error())))
+            th.start()
+            th.join()
+            assert res[0][0], res[0][1]
         else:
             assert leg.lib.vfdTickChems(cc_tick, epi, base, dv, dec, h_fc.data_ptr(), h_fe.data_ptr(), gv, tot), \
                 vfd.last_error()
