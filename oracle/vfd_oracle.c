@@ -210,11 +210,12 @@ int orc_irfft3(int nz, int ny, int nx, const double *in_half, double *out)
 
 /* padDataConstantVF3D (convolutionFFT.cuh:590-638): the interior
  * 0<z<D-1, 0<y<H-1, 0<x<epi copies the field, every other voxel of the whole FFT
- * box is baseChem. */
+ * box is baseChem.  The reference does not bound x by the row length W (an
+ * epi above W reads the next row); here x >= W is baseChem, as outside the data. */
 void orc_pad_constant_vf3d(double *dst, const float *src, int FD, int FH, int FW,
                            int D, int H, int W, int epi, double base)
 {
-    (void)W;
+    if (epi > W) epi = W;
     for (int z = 0; z < FD; z++)
         for (int y = 0; y < FH; y++)
             for (int x = 0; x < FW; x++) {

@@ -68,6 +68,8 @@ def test_native_gauss_f64(vfd, orc, shape):
     ((7, 9, 10), (16, 12, 16), 8, 0.25),
     ((6, 5, 12), (8, 16, 24), 12, 1.5),
     ((11, 13, 14), (24, 32, 36), 9, 0.0),
+    ((6, 7, 8), (12, 16, 16), 11, 0.75),     # epiBoundary beyond the row: bounded at W
+    ((6, 7, 8), (12, 16, 16), 0, 0.5),       # no interior: the whole box is baseChem
 ])
 def test_constant_vf_complex_spectrum(vfd, orc, shape, box, epi, base):
     """Reference semantics: CONSTANT_VF pad, complex R2C-layout spectrum,

@@ -85,10 +85,12 @@ def test_pad_constant_vf_known_answer(orc):
     assert yy.min() == 1 and yy.max() == H - 2
     assert xx.min() == 1 and xx.max() == epi - 1
     assert p[2, 3, 2] == u[2, 3, 2] and p[1, 1, 3] == u[1, 1, 3]
-    # epi beyond the data width keeps x up to W-2... the rule is x < epi only,
-    # the data index stays inside W because epi <= W in the reference callers
+    # epi = W keeps x = 1..W-1; an epi beyond the row (where the reference reads
+    # the next row) is bounded at W: voxels past the row are baseChem
     p2 = orc.pad_constant_vf(u, box, W, 0.0)
     assert (p2 != 0).sum() == (D - 2) * (H - 2) * (W - 1)
+    p3 = orc.pad_constant_vf(u, box, W + 2, 0.0)
+    assert np.array_equal(p3, p2)
 
 
 def test_pad_clamp_known_answer(orc):
