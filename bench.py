@@ -77,6 +77,8 @@ def parse():
                     choices=["step", "ensemble", "serve", "copy", "precision", "update", "dropin"])
     ap.add_argument("--ticks", type=int, default=100, help="precision: ticks of FP32 and FP64 side by side")
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
+    ap.add_argument("--grids", default=None,
+                    help="ensemble: indices into the grid set to draw runs from (default: all five)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
@@ -472,7 +474,8 @@ def bench_ensemble(args):
     import vfd.engine as ve
     rng = np.random.default_rng(args.seed)
     nf = args.fields
-    picks = rng.integers(0, len(ENSEMBLE_GRIDS), args.runs)
+    allowed = [int(g) for g in args.grids.split(",")] if args.grids else list(range(len(ENSEMBLE_GRIDS)))
+    picks = np.array(allowed)[rng.integers(0, len(allowed), args.runs)]
     eng = ve.Engine(int(250e9))
     runs, vox = [], 0.0
     t0 = time.perf_counter()
