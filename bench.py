@@ -639,9 +639,9 @@ def synth_fields(shape, nfields, seed, device):
     return fields, params
 
 
-def cpu_baseline(shape, seed, nfields=4):
+def cpu_baseline(shape, seed, nfields=5):
     """Oracle FP32 OpenMP port on the first fields of the same workload, one
-    tick each (about 10 s of host work at 1024x512x128)."""
+    tick each (about 11 s of host work at 1024x512x128 on 16 threads)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import orc
