@@ -15,7 +15,7 @@ struct Plan {
     int L, hx;
     int rs_y, rs_z, qy, qz;
     bool full_box;
-    long ld, plane, plane_pad;
+    long ld, plane;
     long ks_row, ks_plane;
     size_t work_bytes, kspec_bytes, field_bytes;
     double bytes_per_field_tick;

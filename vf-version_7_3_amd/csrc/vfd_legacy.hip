@@ -270,7 +270,7 @@ bool hip_ok(hipError_t e, const char* what) { return hip_check(e, what) == VFD_O
 // The persistent workspace of a (device, data box, FFT box): the plan and one
 // work array, made on first use.  The plan's schedule is autotuned then (about
 // a second, once per box and process) unless VFD_SCHEDULE fixes it or
-// VFD_AUTOTUNE=0 keeps the default.
+// VFD_NO_AUTOTUNE is set (as for the engine's shapes).
 int ws_for(int dev, const c_ctx& cctx, Ws** out) {
     const WsKey key{dev, cctx.DD, cctx.DH, cctx.DW, cctx.FFTD, cctx.FFTH, cctx.FFTW};
     auto it = g_ws.find(key);
@@ -280,9 +280,7 @@ int ws_for(int dev, const c_ctx& cctx, Ws** out) {
         Ws ws;
         if (int rc = plan_init(&d, &ws.plan)) return rc;
         if (int rc = hip_check(hipMalloc(&ws.work, ws.plan.work_bytes), "legacy workspace")) return rc;
-        const char* fixed = getenv("VFD_SCHEDULE");
-        const char* tune = getenv("VFD_AUTOTUNE");
-        if (!fixed && !(tune && atoi(tune) == 0)) {
+        if (!getenv("VFD_SCHEDULE") && !getenv("VFD_NO_AUTOTUNE")) {
             // timed on a stream of its own, away from the caller's work
             hipStream_t ts = nullptr;
             int rc = hip_check(hipStreamCreateWithFlags(&ts, hipStreamNonBlocking), "autotune stream");

@@ -119,11 +119,6 @@ void tuned_lookup(Plan& q) {
 // ------------------------------------------------------------------ plans
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
-static long env_long(const char* name, long dflt) {
-    const char* s = getenv(name);
-    return (s && *s) ? strtol(s, nullptr, 10) : dflt;
-}
-
 // Smallest FFT length >= 2n that a pass can run: even with a compiled half
 // length for x (R2C via the half-length complex FFT); for y and z a compiled
 // half length (even/odd split) or a compiled full length.
@@ -187,10 +182,9 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     // rows start on 128-byte lines for both the complex work rows and the real
     // kernel-spectrum rows (32 elements)
     q.ld = q.full_box ? q.hx : round_up(q.hx, 32);
-    // Plane strides that are multiples of a large power of two put every row
-    // of a z-column on the same HBM channel; a small pad spreads them.
-    q.plane_pad = q.full_box ? 0 : env_long("VFD_PLANE_PAD", 0);
-    q.plane = (long)q.py * q.ld + q.plane_pad;
+    // (padding the plane stride by 16 to 2064 elements to spread the z-column
+    // rows over HBM channels was measured: no gain)
+    q.plane = (long)q.py * q.ld;
     const size_t cxb = 2 * (size_t)q.prec;
     q.work_bytes = (size_t)q.nz * q.plane * cxb;
     if (q.kspec_format == VFD_KSPEC_REAL) {
