@@ -115,9 +115,12 @@ def test_r2c_c2r_roundtrip(vfd, box):
 
 @pytest.mark.parametrize("shape", [(16, 20, 24), (12, 24, 48), (32, 64, 64)])
 @pytest.mark.parametrize("precision", [4, 8])
-def test_separable_spectrum_bit_identical_to_real(vfd, orc, shape, precision):
-    """VFD_KSPEC_SEPARABLE (FP64 tables, K^ formed in the z pass) gives the
-    same bits as the streamed 3D real spectrum, and matches the oracle."""
+def test_separable_spectrum_matches_real(vfd, orc, shape, precision):
+    """VFD_KSPEC_SEPARABLE against the streamed 3D real spectrum.  FP64: the z
+    pass forms K^ from the FP64 tables and the ticks are bit-identical.  FP32:
+    the z pass applies the z window as a direct convolution (the same operator:
+    gz is the DFT of that window), so the two agree to FP32 rounding, and both
+    match the oracle."""
     import torch
     print(f"case separable_vs_real shape={shape} precision={precision}", flush=True)
     dt = torch.float32 if precision == 4 else torch.float64
@@ -140,7 +143,12 @@ def test_separable_spectrum_bit_identical_to_real(vfd, orc, shape, precision):
     torch.cuda.synchronize()
     for f in range(3):
         a, b = ent_r[f]["out"].cpu().numpy(), ent_s[f]["out"].cpu().numpy()
-        assert np.array_equal(a, b), f"field {f}"
+        if precision == 8:
+            assert np.array_equal(a, b), f"field {f}"
+        else:
+            rel = float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(a.astype(np.float64)))
+            print(f"  field {f}: separable (direct z) vs real (FFT z) rel-L2 {rel:.3e}", flush=True)
+            assert rel <= 1e-6, f"field {f}"
         assert _report(f"field {f}", b, refs[f]) <= (REL_F32 if precision == 4 else REL_F64)
 
 

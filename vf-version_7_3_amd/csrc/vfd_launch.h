@@ -24,4 +24,9 @@ template <typename T>
 hipError_t launch_col(int Q, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                       const void* tw, hipStream_t st, int variant = 0);
 
+// direct z pass (FP32, separable spectrum) for window radius R <= rm, rm one of
+// 8, 12, 16 (kZDirectMaxR); cd.scale carries Pz.
+hipError_t launch_zdirect(int rm, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
+                          hipStream_t st);
+
 }  // namespace vfd

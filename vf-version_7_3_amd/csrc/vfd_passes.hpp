@@ -533,4 +533,69 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     }
 }
 
+// ------------------------------------------------ z pass, direct form (FP32)
+// With the separable spectrum the z factor gz is the DFT, on the Pz box, of
+// the normalized window g_j (|j| <= R).  Multiplying by gz between the forward
+// and the inverse z transforms is therefore the linear convolution of each
+// column with that window (the box is at least nz + R long, nothing wraps),
+// and this pass applies it directly: per column (kx, ky)
+//   y[n] = ((gx * gy) * scale * Pz) * sum_{|j| <= R} g_|j| x[n - j],  n < nz,
+// with x zero outside [0, nz).  Same operator as the FFT z pass, with
+// 4 RM + 2 flops per output instead of four length-Pz/2 transforms.
+//
+// One lane per column, 64 consecutive columns per wave (512-byte rows).  A
+// lane streams its column once: the window of 2 RM + 1 values lives in
+// registers, rotated at compile time inside blocks of U = 2 RM + 1 outputs,
+// and row n + RM is loaded before row n is written, so the pass is in place.
+// RM is a compile-time bound on R; the table holds zero taps beyond R.
+
+template <int RM>
+__global__ void __launch_bounds__(64)
+zdirect_kernel(const FieldBatch fb, ColDesc cd) {
+    using B = BufOps<float>;
+    using O = typename B::O;
+    using BD = BufOps<double>;
+    using OD = typename BD::O;
+    constexpr int U = 2 * RM + 1;
+    constexpr unsigned CB = sizeof(cx<float>);
+    const FieldDesc& fd = fb.f[blockIdx.z];
+    const auto rwork = B::make(fd.work, cd.work_bytes);
+    const long col = (long)blockIdx.x * 64 + threadIdx.x;
+    const bool live = col < cd.ncols;
+    const O base = (O)(((long)blockIdx.y * cd.pstride + (live ? col : 0)) * CB);
+    const O rstep = (O)cd.rstride * CB;
+    const int nz = cd.nin;
+    const auto tk = BD::make(fd.kspec, cd.kspec_bytes);
+    float g[RM + 1];
+#pragma unroll
+    for (int j = 0; j <= RM; ++j) g[j] = (float)BD::ld_r(tk, (OD)(cd.sep_g + 1 + j) * 8u);
+    const double sc = BD::ld_r(tk, OD(0));
+    const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + blockIdx.y) * 8u);
+    const double gxv = BD::ld_r(tk, live ? (OD)(1 + col) * 8u : BD::OOB);
+    const float ka = (float)(((gxv * gyv) * sc) * cd.scale);  // cd.scale = Pz here
+    auto row = [&](int m) -> O { return (live && m >= 0 && m < nz) ? base + (O)m * rstep : B::OOB; };
+    // w[(n - RM + k) mod U] holds x[n - RM + k]; x[-RM .. RM] to start
+    cx<float> w[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = B::ld_cx(rwork, row(k - RM));
+    for (int n0 = 0; n0 < nz; n0 += U) {
+#pragma unroll
+        for (int i = 0; i < U; ++i) {
+            // output n = n0 + i; its centre x[n] sits at slot (i + RM) mod U
+            const int n = n0 + i;
+            const int c = (i + RM) % U;
+            cx<float> acc{g[0] * w[c].x, g[0] * w[c].y};
+#pragma unroll
+            for (int j = 1; j <= RM; ++j) {
+                const cx<float> a = w[(c + U - j) % U], b = w[(c + j) % U];
+                acc.x = fmaf(g[j], a.x + b.x, acc.x);
+                acc.y = fmaf(g[j], a.y + b.y, acc.y);
+            }
+            // x[n - RM] is no longer needed: its slot takes x[n + RM + 1]
+            w[i] = B::ld_cx(rwork, row(n + RM + 1));
+            B::template st_cx<kStream>(cx<float>{ka * acc.x, ka * acc.y}, rwork, row(n));
+        }
+    }
+}
+
 }  // namespace vfd

@@ -12,6 +12,7 @@
 #include <array>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -194,10 +195,11 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
         q.ks_plane = q.ld * q.rs_z;             // per ky
         q.kspec_bytes = (size_t)q.pz * q.py * q.ld * q.prec;
     } else if (q.kspec_format == VFD_KSPEC_SEPARABLE) {
-        // FP64 tables [scale | gx[hx] | gy[py] | gz[pz]]
+        // FP64 tables [scale | gx[hx] | gy[py] | gz[pz] | R | g_0 .. g_kZDirectMaxR]
+        // (the last part: the z window's radius and taps for the direct z pass)
         q.ks_row = 0;
         q.ks_plane = 0;
-        q.kspec_bytes = round_up((long)(1 + q.hx + q.py + q.pz), 32) * sizeof(double);
+        q.kspec_bytes = round_up((long)(1 + q.hx + q.py + q.pz + 2 + kZDirectMaxR), 32) * sizeof(double);
     } else {
         q.ks_row = (long)q.py * q.hx;
         q.ks_plane = q.hx;
@@ -236,6 +238,55 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     if (int rc = twiddles(q.pz, q.prec, &q.tw_z)) return rc;
     *p = q;
     return VFD_OK;
+}
+
+// Window radius of each separable table plan_gauss_spectrum wrote (by device
+// address): the z pass runs the direct form for R <= kZDirectMaxR.  A table
+// this process did not write runs the FFT form.
+static std::mutex g_sepr_mu;
+static std::unordered_map<const void*, int> g_sepr;
+static void sep_radius_set(const void* table, int R) {
+    std::lock_guard<std::mutex> lk(g_sepr_mu);
+    g_sepr[table] = R;
+}
+static int sep_radius(const void* table) {
+    std::lock_guard<std::mutex> lk(g_sepr_mu);
+    auto it = g_sepr.find(table);
+    return it == g_sepr.end() ? -1 : it->second;
+}
+// the direct z pass's compiled radius bound for R (0: use the FFT form)
+static int zdirect_bucket(int R) {
+    for (int rm : {8, 12, 16})
+        if (R >= 0 && R <= rm) return rm;
+    return 0;
+}
+
+// z pass: FP32 fields with a separable table of a window radius up to
+// kZDirectMaxR take the direct form, grouped by radius bound (8, 12, 16); the
+// rest the FFT form
+template <typename T>
+static hipError_t launch_z(const Plan& p, const FieldBatch& fb, int nf, const ColDesc& z, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (p.kspec_format == VFD_KSPEC_SEPARABLE) {
+            int bucket[kMaxFieldsPerLaunch];
+            for (int i = 0; i < nf; ++i) bucket[i] = zdirect_bucket(sep_radius(fb.f[i].kspec));
+            ColDesc zd = z;
+            zd.scale = (double)p.pz;
+            for (int rm : {0, 8, 12, 16}) {
+                FieldBatch sub;
+                memset(&sub, 0, sizeof sub);
+                int n = 0;
+                for (int i = 0; i < nf; ++i)
+                    if (bucket[i] == rm) sub.f[n++] = fb.f[i];
+                if (n == 0) continue;
+                const hipError_t e = rm ? launch_zdirect(rm, sub, n, zd, p.py, st)
+                                        : launch_col<T>(p.qz, sub, n, z, p.py, p.tw_z, st, p.var[VFD_PASS_ZCONV]);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
+    }
+    return launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[VFD_PASS_ZCONV]);
 }
 
 // ------------------------------------------------------------- execution
@@ -277,6 +328,7 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
         z.kspec_sep = p.kspec_format == VFD_KSPEC_SEPARABLE;
         z.sep_gy = 1 + p.hx;
         z.sep_gz = 1 + p.hx + p.py;
+        z.sep_g = 1 + p.hx + p.py + p.pz;
         // the real native spectrum carries the 1/(px py pz) normalization
         z.scale = z.kspec_complex ? 1.0 / ((double)p.px * p.py * p.pz) : 1.0;
         z.work_bytes = p.work_bytes; z.kspec_bytes = p.kspec_bytes;
@@ -287,7 +339,7 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             switch (pass) {
                 case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
                 case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st, p.var[pass]); break;
-                case VFD_PASS_ZCONV: e = launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[pass]); break;
+                case VFD_PASS_ZCONV: e = launch_z<T>(p, fb, nf, z, st); break;
                 case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
                 case VFD_PASS_XINV:
                     e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass], epi && epi->vis);
@@ -472,12 +524,19 @@ __global__ void gauss_product_kernel(T* __restrict__ out, const double* __restri
     out[i] = kx < hx ? (T)(((gx[kx] * gy[ky]) * scale) * gz[kz]) : T(0);
 }
 
-static void gauss_dft(double sigma2, int p, int nk, std::vector<double>& out) {
+// the normalized window g_j, |j| <= R = ceil(6 sigma), as g[j + R]
+static void gauss_window(double sigma2, std::vector<double>& g) {
     const int R = (int)std::ceil(6.0 * std::sqrt(sigma2));
-    std::vector<double> g(2 * R + 1);
+    g.assign(2 * R + 1, 0.0);
     double s = 0.0;
     for (int j = -R; j <= R; ++j) s += (g[j + R] = std::exp(-(double)j * j / (2.0 * sigma2)));
     for (auto& v : g) v /= s;
+}
+
+static void gauss_dft(double sigma2, int p, int nk, std::vector<double>& out) {
+    std::vector<double> g;
+    gauss_window(sigma2, g);
+    const int R = ((int)g.size() - 1) / 2;
     out.assign(nk, 0.0);
     const long double tp = 6.283185307179586476925286766559005768L;
     for (int k = 0; k < nk; ++k) {
@@ -512,10 +571,17 @@ int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspe
         std::copy(gx.begin(), gx.end(), tab.begin() + 1);
         std::copy(gy.begin(), gy.end(), tab.begin() + 1 + p.hx);
         std::copy(gz.begin(), gz.end(), tab.begin() + 1 + p.hx + p.py);
+        // the z window for the direct z pass: R, then g_0 .. g_min(R, kZDirectMaxR)
+        std::vector<double> win;
+        gauss_window(sigma2, win);
+        const size_t g0 = 1 + p.hx + p.py + p.pz;
+        tab[g0] = (double)R;
+        for (int j = 0; j <= std::min(R, kZDirectMaxR); ++j) tab[g0 + 1 + j] = win[R + j];
         int rc = hip_check(hipMemcpyAsync(d_kspec, tab.data(), p.kspec_bytes, hipMemcpyHostToDevice, st),
                            "gauss tables upload");
         // the host vector must outlive the async copy
         if (!rc) rc = hip_check(hipStreamSynchronize(st), "gauss tables sync");
+        if (!rc) sep_radius_set(d_kspec, R);
         return rc;
     }
     double* d = nullptr;

@@ -61,8 +61,15 @@ struct ColDesc {
     unsigned long work_bytes, kspec_bytes;  // per-field array sizes
     // grouped 1-D launch of narrow tiles (set by the launcher): planes and
     // fields of the launch, 128-byte row segments per plane row
-    unsigned grid_planes, grid_fields, grid_segs, pad_;
+    unsigned grid_planes, grid_fields, grid_segs;
+    int sep_g;          // separable: table offset (elements) of [R | g_0 .. g_kZDirectMaxR]
 };
+
+// Largest window radius the direct z pass takes (separable spectrum, FP32);
+// the separable table carries the taps g_0 .. g_kZDirectMaxR.  Wider windows
+// (20, 24 taps a side) ran out of registers (scratch spills) and were slower
+// than the FFT form, which they keep.
+constexpr int kZDirectMaxR = 16;
 
 // Every per-field array is addressed with 32-bit byte offsets.
 constexpr unsigned long kBufLimit = 0xFFFFFF00ul;
