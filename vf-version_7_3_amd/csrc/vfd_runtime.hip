@@ -577,6 +577,14 @@ int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspe
         const size_t g0 = 1 + p.hx + p.py + p.pz;
         tab[g0] = (double)R;
         for (int j = 0; j <= std::min(R, kZDirectMaxR); ++j) tab[g0 + 1 + j] = win[R + j];
+        if (R <= kZDirectMaxR) {
+            // the direct pass multiplies by the taps rounded to FP32; g_0 takes
+            // up their rounding so that g_0 + 2 sum_j g_j stays 1 to FP32
+            // precision (otherwise the taps' rounding is a per-tick gain)
+            double tail = 0.0;
+            for (int j = 1; j <= R; ++j) tail += (double)(float)win[R + j];
+            tab[g0 + 1] = 1.0 - 2.0 * tail;
+        }
         int rc = hip_check(hipMemcpyAsync(d_kspec, tab.data(), p.kspec_bytes, hipMemcpyHostToDevice, st),
                            "gauss tables upload");
         // the host vector must outlive the async copy
