@@ -48,10 +48,17 @@ enum { VFD_PAD_CONSTANT_VF = 3, VFD_PAD_ZERO = 100 };
  *  VFD_KSPEC_COMPLEX_STD complex K^ in the R2C layout [pz][py][px/2+1] of the
  *                        reference (d_kernelspectrum_h, Diffusion.cu:1605)
  *  VFD_KSPEC_SEPARABLE   a separable even kernel as FP64 tables
- *                        [scale | gx[px/2+1] | gy[py] | gz[pz]]; the z pass forms
- *                        K^(kz,ky,kx) = ((gx*gy)*scale)*gz in FP64 and rounds it
- *                        to the plan's precision, the same numbers VFD_KSPEC_REAL
- *                        stores for the Gaussian, without streaming a 3D array.
+ *                        [scale | gx[px/2+1] | gy[py] | gz[pz] | R | g_0..g_16],
+ *                        gz being the DFT of the normalized z window g of radius
+ *                        R.  FP64 plans (and FP32 windows with R > 16): the z
+ *                        pass forms K^(kz,ky,kx) = ((gx*gy)*scale)*gz in FP64
+ *                        and rounds it to the plan's precision, the same numbers
+ *                        VFD_KSPEC_REAL stores for the Gaussian.  FP32 plans with
+ *                        R <= 16: the z pass convolves each z column with g
+ *                        directly (the same operator; agrees with the FFT form to
+ *                        FP32 rounding).  The direct form is chosen for tables
+ *                        vfd_gauss_spectrum wrote, by their device address: write
+ *                        a separable table only through vfd_gauss_spectrum.
  *                        VFD_PAD_ZERO plans only. */
 enum { VFD_KSPEC_REAL = 0, VFD_KSPEC_COMPLEX_STD = 1, VFD_KSPEC_SEPARABLE = 2 };
 
@@ -121,7 +128,7 @@ int vfd_plan_set_schedule(vfd_plan* plan, const int* variants);  /* 5 ints, 0..3
  * |j| <= ceil(6 sqrt(sigma2)) per axis, times decay, with 1/(px py pz) folded
  * in: the 3D array in the plan's precision (VFD_KSPEC_REAL) or the FP64 tables
  * (VFD_KSPEC_SEPARABLE), computed in FP64.  Both formats give bit-identical
- * ticks. */
+ * ticks in FP64 and agree to FP32 rounding in FP32 (direct z pass). */
 int vfd_gauss_spectrum(const vfd_plan* plan, double sigma2, double decay, void* d_kspec, void* stream);
 
 /* Compensated (Neumaier, FP64) sum of each of nfields real fields of the plan's
