@@ -67,8 +67,8 @@ struct ColDesc {
 
 // Largest window radius the direct z pass takes (separable spectrum, FP32);
 // the separable table carries the taps g_0 .. g_kZDirectMaxR.  Wider windows
-// (20, 24 taps a side) ran out of registers (scratch spills) and were slower
-// than the FFT form, which they keep.
+// (20, 24 taps a side) compiled to scratch-memory window accesses and were
+// slower than the FFT form, which they keep.
 constexpr int kZDirectMaxR = 16;
 
 // Every per-field array is addressed with 32-bit byte offsets.
