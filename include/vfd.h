@@ -56,9 +56,9 @@ enum { VFD_PAD_CONSTANT_VF = 3, VFD_PAD_ZERO = 100 };
  *                        VFD_KSPEC_REAL stores for the Gaussian.  FP32 plans with
  *                        R <= 16: the z pass convolves each z column with g
  *                        directly (the same operator; agrees with the FFT form to
- *                        FP32 rounding).  The direct form is chosen for tables
- *                        vfd_gauss_spectrum wrote, by their device address: write
- *                        a separable table only through vfd_gauss_spectrum.
+ *                        FP32 rounding).  The form is chosen on the device from
+ *                        the R stored in the table, so a table copied or moved
+ *                        by any means runs the form its own R selects.
  *                        VFD_PAD_ZERO plans only. */
 enum { VFD_KSPEC_REAL = 0, VFD_KSPEC_COMPLEX_STD = 1, VFD_KSPEC_SEPARABLE = 2 };
 

@@ -174,3 +174,30 @@ def test_release_workspaces_with_nothing_held(lib):
     """vfdReleaseWorkspaces with no workspace made is a no-op returning 0 (no device call)."""
     lib.vfdReleaseWorkspaces.restype = ctypes.c_int
     assert lib.vfdReleaseWorkspaces() == 0
+
+
+def test_convolution_clamp_cpu_definition(lib):
+    """convolutionClampToBorderCPU (a host routine of the library) against its
+    definition evaluated independently with numpy: clamped (edge-replicated)
+    data shifted per kernel tap, each product rounded to FP32, summed in FP64."""
+    import numpy as np
+    rng = np.random.default_rng(5)
+    H, W, kH, kW, KY, KX = 7, 9, 3, 5, 1, 3          # off-centre window, clamps on every side
+    data = rng.standard_normal((H, W)).astype(np.float32)
+    k = rng.standard_normal((kH, kW)).astype(np.float32)
+    res = np.zeros_like(data)
+    P = ctypes.c_void_p
+    lib.convolutionClampToBorderCPU(P(res.ctypes.data), P(data.ctypes.data), P(k.ctypes.data), H, W, kH, kW, KY, KX)
+    padded = np.pad(data, ((kH, kH), (kW, kW)), mode="edge")
+    acc = np.zeros((H, W), np.float64)
+    for r in range(kH):
+        for c in range(kW):
+            # u(clamp(y + KY - r), clamp(x + KX - c)) for every (y, x)
+            sh = padded[kH + KY - r: kH + KY - r + H, kW + KX - c: kW + KX - c + W]
+            acc += (sh * k[r, c]).astype(np.float32).astype(np.float64)
+    assert np.allclose(res, acc.astype(np.float32), rtol=1e-6, atol=1e-6)
+    # identity kernel: the data itself, exactly
+    kid = np.zeros((kH, kW), np.float32)
+    kid[KY, KX] = 1.0
+    lib.convolutionClampToBorderCPU(P(res.ctypes.data), P(data.ctypes.data), P(kid.ctypes.data), H, W, kH, kW, KY, KX)
+    assert np.array_equal(res, data)

@@ -227,6 +227,35 @@ def test_2d_pads_known_answers(vfd, leg):
     assert np.all(got[2][:3] == np.float32(0.0001))   # constant band below the data
 
 
+@pytest.mark.parametrize("fH,fW,H,W,KY,KX", [(9, 11, 3, 4, 2, 2), (8, 13, 3, 4, 1, 6), (7, 7, 2, 2, 4, 1)])
+def test_right_wall_pad_case_table(vfd, leg, fH, fW, H, W, KY, KX):
+    """padDataRightWall over the whole box against its case table: data; to
+    the right a band mirrored in x (rows: data, then mirrored in y, then the
+    far-side mirror fH-1-y); 0.0001 everywhere else and where a mirror index
+    falls outside the data (bands wider than the data in the second and third
+    cases, far-side rows beyond the data in the first)."""
+    import torch
+    print(f"case right_wall box={fH}x{fW} data={H}x{W} centre=({KY},{KX})", flush=True)
+    src = (np.arange(H * W, dtype=np.float32) + 1).reshape(H, W)
+    ds = _t(src)
+    out = torch.empty((fH, fW), device="cuda")
+    P = ctypes.c_void_p
+    leg.lib.padDataRightWall(P(out.data_ptr()), P(ds.data_ptr()), fH, fW, H, W, 3, 3, KY, KX)
+    torch.cuda.synchronize()
+    c = np.float32(0.0001)
+    expect = np.full((fH, fW), c, np.float32)
+    ys, xs = np.arange(fH), np.arange(fW)
+    expect[:H, :W] = src
+    band_x = xs[(xs >= W) & (xs < W + KX)]
+    for y in ys:
+        sy = y if y < H else (2 * H - 1 - y if y < H + KY else fH - 1 - y)
+        for x in band_x:
+            sx = 2 * W - 1 - x
+            if 0 <= sy < H and sx >= 0:
+                expect[y, x] = src[sy, sx]
+    assert np.array_equal(out.cpu().numpy(), expect)
+
+
 def test_convolution_clamp_cpu(vfd, leg):
     """convolutionClampToBorderCPU (host): identity kernel and clamp edges."""
     data = np.arange(12, dtype=np.float32).reshape(3, 4)

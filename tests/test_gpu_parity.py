@@ -306,3 +306,69 @@ def test_more_than_32_fields_and_empty_batch(vfd, orc):
     torch.cuda.synchronize()
     for f in range(40):
         assert orc.rel_l2(ents[f]["out"].cpu().numpy(), refs[f]) <= REL_F32, f
+
+
+@pytest.mark.parametrize("shape,box,epi,base,s2", [
+    ((11, 13, 14), (24, 32, 36), 9, 0.75, 1.0),
+    ((7, 9, 10), (16, 24, 24), 8, 2.5, 2.2),
+])
+def test_constant_vf_real_spectrum(vfd, orc, shape, box, epi, base, s2):
+    """CONSTANT_VF pad with the native real spectrum, whose 1/(FD FH FW) and
+    decay are folded into the stored values: the restore term base * sum(k)
+    must come out as base * decay (the oracle's complex spectrum is stored
+    unnormalized, like the reference's)."""
+    import torch
+    print(f"case constant_vf_real shape={shape} box={box} base={base}", flush=True)
+    dec = 0.93
+    u = orc.synth_field(shape, 78, nsrc=3)
+    win = orc.gauss_window3d(s2, dec)
+    R = (win.shape[0] - 1) // 2
+    spec = orc.kernel_spectrum(win, (R, R, R), box)
+    ref = orc.fft_diffuse3d(u, spec, box, orc.PAD_CONSTANT_VF, epi, base)
+    plan = vfd.Plan(shape, box, vfd.F32, vfd.PAD_CONSTANT_VF, vfd.KSPEC_REAL)
+    ud = torch.from_numpy(u).cuda()
+    out = torch.empty_like(ud)
+    ks = plan.gauss_spectrum(s2, dec)
+    plan.diffuse([{"in": ud, "out": out, "work": plan.new_work(), "kspec": ks, "base": base, "epi": epi}])
+    torch.cuda.synchronize()
+    assert _report("cvf real", out.cpu().numpy(), ref) <= REL_F32
+
+
+def test_separable_table_at_reused_address(vfd, orc):
+    """The z-pass form follows the radius each separable table stores, not
+    where the table lives: a table copied into memory that held a table of
+    another radius (as the caching allocator hands back), and a fresh copy at
+    a new address, both match the oracle."""
+    import torch
+    shape = (32, 64, 64)
+    print(f"case separable_table_reused shape={shape}", flush=True)
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE)
+    u = orc.synth_field(shape, 4321, nsrc=5)
+    ud = torch.from_numpy(u).cuda()
+    work = plan.new_work()
+    small, large = (1.0, 0.98), (9.0, 0.95)      # radius 6 (direct z pass), 18 (FFT z pass)
+    assert orc.gauss_radius(small[0]) <= 8 and orc.gauss_radius(large[0]) > 16
+    slot = plan.gauss_spectrum(*small)            # this memory first holds the radius-6 table
+    cases = []
+    for name, (s2, dec) in (("large into small's memory", large), ("small into large's memory", small)):
+        src = plan.gauss_spectrum(s2, dec)
+        slot.copy_(src)
+        del src
+        cases.append((name, slot, s2, dec))
+        out = torch.empty_like(ud)
+        plan.diffuse([{"in": ud, "out": out, "work": work, "kspec": slot}])
+        torch.cuda.synchronize()
+        assert _report(name, out.cpu().numpy(), orc.direct_separable(u, s2, dec)) <= REL_F32
+    fresh = slot.clone()                          # a copy at an address no table was written to
+    out = torch.empty_like(ud)
+    plan.diffuse([{"in": ud, "out": out, "work": work, "kspec": fresh}])
+    torch.cuda.synchronize()
+    assert _report("fresh copy", out.cpu().numpy(), orc.direct_separable(u, *small)) <= REL_F32
+    # both forms in one launch: a batch mixing the two radii
+    ks = [plan.gauss_spectrum(*small), plan.gauss_spectrum(*large)]
+    outs = [torch.empty_like(ud), torch.empty_like(ud)]
+    works = [work, plan.new_work()]
+    plan.diffuse([{"in": ud, "out": outs[i], "work": works[i], "kspec": ks[i]} for i in range(2)])
+    torch.cuda.synchronize()
+    for i, (s2, dec) in enumerate((small, large)):
+        assert _report(f"mixed batch {i}", outs[i].cpu().numpy(), orc.direct_separable(u, s2, dec)) <= REL_F32

@@ -178,39 +178,44 @@ __global__ void pad_kernel2d(float* dst, const float* src, int fH, int fW, int k
     dst[(long)ky * fW + kx] = src[i];
 }
 
-// padDataConstantVF_kernel (convolutionFFT.cuh:179-218)
+// padDataConstantVF_kernel (convolutionFFT.cuh:179-218); like the 3D pad, an
+// epithelium index beyond the row is bounded at W (the reference reads the
+// next rows there)
 __global__ void pad_constant_vf2d(float* dst, const float* src, int fH, int fW, int H, int W, int epi,
                                   float base) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)fH * fW) return;
     const int x = (int)(i % fW), y = (int)(i / fW);
-    const bool in = y < H - 1 && y > 0 && x < epi && x > 0;
+    const bool in = y < H - 1 && y > 0 && x < epi && x < W && x > 0;
     dst[i] = in ? src[(long)y * W + x] : base;
 }
 
-// padDataRightWall_kernel (convolutionFFT.cuh:223-310): data, a constant band
-// (0.0001) below the data, mirror to the right, mirror-of-fft below-right.
+// padDataRightWall (convolutionFFT.cuh:261-336), restated from its case table.
+// Each axis of the FFT box splits into the data [0, N), a band [N, N + K) of
+// the kernel centre's width, and the rest.  With the mirrors mx = 2W - 1 - x,
+// my = 2H - 1 - y and the far-side mirror fy = fH - 1 - y, the source is
+//                    x < W      W <= x < W + KX    x >= W + KX
+//   y < H            (y, x)     (y, mx)            c
+//   H <= y < H + KY  c          (my, mx)           c
+//   y >= H + KY      c          (fy, mx)           c
+// and c = 0.0001 also where a mirror falls outside the data: below 0 for a
+// band wider than the data, fy >= H for a box taller than 2H + KY (there the
+// reference reads outside the field).
 __global__ void pad_right_wall2d(float* dst, const float* src, int fH, int fW, int H, int W, int KY, int KX) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)fH * fW) return;
     const int x = (int)(i % fW), y = (int)(i / fW);
-    const int bH = H + KY, bW = W + KX;
-    int dy = -1, dx = -1;
-    do {
-        if (y < H) dy = y;
-        if (x < W) {
-            dx = x;
-            if (y >= H && y < bH) { dy = -1; break; }
-        }
-        if (y >= H && y < bH) dy = H - (y - H) - 1;
-        if (x >= W && x < bW) {
-            dx = W - (x - W) - 1;
-            if (y >= bH) { dy = fH - y - 1; break; }
-        }
-        if (y >= bH) dy = -1;
-        if (x >= bW) dx = -1;
-    } while (0);
-    dst[i] = (dx < 0 || dy < 0) ? 0.0001f : src[(long)dy * W + dx];
+    const int row_region = y < H ? 0 : (y < H + KY ? 1 : 2);
+    const int col_region = x < W ? 0 : (x < W + KX ? 1 : 2);
+    int sy = -1, sx = -1;
+    if (col_region == 0 && row_region == 0) {
+        sy = y;
+        sx = x;
+    } else if (col_region == 1) {
+        sx = 2 * W - 1 - x;
+        sy = row_region == 0 ? y : (row_region == 1 ? 2 * H - 1 - y : fH - 1 - y);
+    }
+    dst[i] = (sx < 0 || sy < 0 || sy >= H) ? 0.0001f : src[(long)sy * W + sx];
 }
 
 // padDataMirror_kernel (convolutionFFT.cuh:315-363)
@@ -226,6 +231,11 @@ __global__ void pad_mirror2d(float* dst, const float* src, int fH, int fW, int H
     if (x >= W && x < bW) dx = W - (x - W) - 1;
     if (y >= bH) dy = fH - y - 1;
     if (x >= bW) dx = fW - x - 1;
+    // a mirror that falls outside the data (a band wider than the data, a box
+    // beyond 2N + K: the reference reads outside the field there) takes the
+    // nearest data index
+    dy = dy < 0 ? 0 : (dy >= H ? H - 1 : dy);
+    dx = dx < 0 ? 0 : (dx >= W ? W - 1 : dx);
     dst[i] = src[(long)dy * W + dx];
 }
 
@@ -1031,22 +1041,30 @@ void modulateAndNormalize(fComplex* d_Dst, fComplex* d_Src, int fftH, int fftW, 
     hip_ok(hipGetLastError(), "modulateAndNormalize");
 }
 
-// convolutionFFT_gold.cpp:22-58: FP64-accumulated direct 2D convolution with
-// clamp-to-border addressing.
+// convolutionClampToBorderCPU (convolutionFFT_gold.cpp:22-58), restated from
+// its definition.  For kernel taps k[r][c] (r < kH, c < kW) centred at
+// (KY, KX) and clampN(i) = min(max(i, 0), N - 1):
+//   out(y, x) = sum over r = kH-1 down to 0, then c = kW-1 down to 0, of
+//               fp32( u(clampH(y + KY - r), clampW(x + KX - c)) * k[r][c] ),
+// each product rounded to FP32 and summed in FP64 in that order, the sum
+// rounded to FP32.  (The order and the product's rounding decide the last bits.)
 void convolutionClampToBorderCPU(float* h_Result, float* h_Data, float* h_Kernel, int dataH, int dataW, int kernelH,
                                  int kernelW, int kernelY, int kernelX) {
-    for (int y = 0; y < dataH; y++)
-        for (int x = 0; x < dataW; x++) {
-            double sum = 0;
-            for (int ky = -(kernelH - kernelY - 1); ky <= kernelY; ky++)
-                for (int kx = -(kernelW - kernelX - 1); kx <= kernelX; kx++) {
-                    int dy = y + ky, dx = x + kx;
-                    dy = dy < 0 ? 0 : (dy >= dataH ? dataH - 1 : dy);
-                    dx = dx < 0 ? 0 : (dx >= dataW ? dataW - 1 : dx);
-                    sum += (double)h_Data[dy * dataW + dx] * (double)h_Kernel[(kernelY - ky) * kernelW + (kernelX - kx)];
-                }
-            h_Result[y * dataW + x] = (float)sum;
+    auto clampi = [](int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); };
+    const long n = (long)dataH * dataW;
+    for (long o = 0; o < n; ++o) {
+        const int y = (int)(o / dataW), x = (int)(o % dataW);
+        double acc = 0.0;
+        for (int r = kernelH - 1; r >= 0; --r) {
+            const float* urow = h_Data + (long)clampi(y + kernelY - r, dataH) * dataW;
+            const float* krow = h_Kernel + (long)r * kernelW;
+            for (int c = kernelW - 1; c >= 0; --c) {
+                const float prod = urow[clampi(x + kernelX - c, dataW)] * krow[c];
+                acc += (double)prod;
+            }
         }
+        h_Result[o] = (float)acc;
+    }
 }
 
 }  // extern "C"

@@ -225,7 +225,9 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         if (g.pad_mode == PAD_CONSTANT_VF) {
             const auto rk = B::make(fd.kspec, g.kspec_bytes);
             const T dc = B::ld_r(rk, 0);  // Re K^(0,0,0) (real part first in both layouts)
-            addc = (T)fd.base * dc;
+            // base * sum(k): the complex spectrum holds sum(k) itself; the real
+            // one sum(k) / (Px Py Pz), which kdc_scale undoes
+            addc = g.kspec_complex ? (T)fd.base * dc : (T)((double)fd.base * ((double)dc * g.kdc_scale));
         }
         const O orow = (O)(row * g.Nx) * RB;
         const bool vec = (g.Nx % 2 == 0);
@@ -315,6 +317,25 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
 // layout, scaled by cd.scale), or KS_SEP (FP64 tables, K^ formed here).
 enum { KS_REAL = 0, KS_COMPLEX = 1, KS_SEP = 2 };
 
+// Which z-pass form an FP32 field with a separable table takes, from the
+// window radius R the table itself stores (tab[cd.sep_g]): 8, 12 or 16 -- the
+// direct form compiled for radius bound 8 (R in [0, 8]), 12 (R in (8, 12]) or
+// 16 (R in (12, 16]) -- or 0, the FFT form (R > 16, or no valid radius).
+// Every z-pass launch of a separable batch covers all its fields and each
+// work-group keeps only the fields of its own form, so the choice follows the
+// table's contents wherever the table came from.
+__device__ __forceinline__ int zform_of(double rd) {
+    if (rd >= 0.0 && rd <= 8.0) return 8;
+    if (rd > 8.0 && rd <= 12.0) return 12;
+    if (rd > 12.0 && rd <= (double)kZDirectMaxR) return 16;
+    return 0;  // also NaN
+}
+__device__ __forceinline__ int zform_of_table(const void* kspec, unsigned long kspec_bytes, int sep_g) {
+    using BD = BufOps<double>;
+    const auto tk = BD::make(kspec, kspec_bytes);
+    return zform_of(BD::ld_r(tk, (typename BD::O)sep_g * 8u));
+}
+
 template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 __global__ void __launch_bounds__((ColCfg<T, Q, V, RS>::NT))
 col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
@@ -346,6 +367,11 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
         if (bz >= cd.grid_fields || (long)bx * C::TC >= cd.ncols) return;
     }
     const FieldDesc& fd = fb.f[bz];
+    if constexpr (SEP && sizeof(T) == 4) {
+        // FP32 separable tables of radius <= kZDirectMaxR take the direct z pass
+        // (zdirect_kernel, launched over the same fields); whole-block exit
+        if (zform_of_table(fd.kspec, cd.kspec_bytes, cd.sep_g) != 0) return;
+    }
     const auto rwork = B::make(fd.work, cd.work_bytes);
     const int c = threadIdx.x % C::TC;
     const int t = threadIdx.x / C::TC;
@@ -547,7 +573,9 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
 // lane streams its column once: the window of 2 RM + 1 values lives in
 // registers, rotated at compile time inside blocks of U = 2 RM + 1 outputs,
 // and row n + RM is loaded before row n is written, so the pass is in place.
-// RM is a compile-time bound on R; the table holds zero taps beyond R.
+// RM is a compile-time bound on R; the table holds zero taps beyond R.  A
+// launch covers every field of the batch: a work-group whose table's radius
+// is not in this RM's range (zform_of) exits at once.
 
 template <int RM>
 __global__ void __launch_bounds__(64)
@@ -559,6 +587,7 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
     constexpr int U = 2 * RM + 1;
     constexpr unsigned CB = sizeof(cx<float>);
     const FieldDesc& fd = fb.f[blockIdx.z];
+    if (zform_of_table(fd.kspec, cd.kspec_bytes, cd.sep_g) != RM) return;
     const auto rwork = B::make(fd.work, cd.work_bytes);
     const long col = (long)blockIdx.x * 64 + threadIdx.x;
     const bool live = col < cd.ncols;

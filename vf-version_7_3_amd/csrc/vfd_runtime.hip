@@ -12,7 +12,6 @@
 #include <array>
 #include <map>
 #include <mutex>
-#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -240,50 +239,20 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     return VFD_OK;
 }
 
-// Window radius of each separable table plan_gauss_spectrum wrote (by device
-// address): the z pass runs the direct form for R <= kZDirectMaxR.  A table
-// this process did not write runs the FFT form.
-static std::mutex g_sepr_mu;
-static std::unordered_map<const void*, int> g_sepr;
-static void sep_radius_set(const void* table, int R) {
-    std::lock_guard<std::mutex> lk(g_sepr_mu);
-    g_sepr[table] = R;
-}
-static int sep_radius(const void* table) {
-    std::lock_guard<std::mutex> lk(g_sepr_mu);
-    auto it = g_sepr.find(table);
-    return it == g_sepr.end() ? -1 : it->second;
-}
-// the direct z pass's compiled radius bound for R (0: use the FFT form)
-static int zdirect_bucket(int R) {
-    for (int rm : {8, 12, 16})
-        if (R >= 0 && R <= rm) return rm;
-    return 0;
-}
-
-// z pass: FP32 fields with a separable table of a window radius up to
-// kZDirectMaxR take the direct form, grouped by radius bound (8, 12, 16); the
-// rest the FFT form
+// z pass.  FP32 fields with a separable table take the direct form when the
+// table's window radius R is at most kZDirectMaxR, else the FFT form.  The
+// form is decided on the device from the R the table stores: the direct
+// launches (radius bounds 8, 12, 16) and the FFT launch each cover every field
+// of the batch, and a work-group whose field is not of its form exits at once
+// (zform_of in vfd_passes.hpp), so no host-side record of the tables is needed.
 template <typename T>
 static hipError_t launch_z(const Plan& p, const FieldBatch& fb, int nf, const ColDesc& z, hipStream_t st) {
     if constexpr (sizeof(T) == 4) {
         if (p.kspec_format == VFD_KSPEC_SEPARABLE) {
-            int bucket[kMaxFieldsPerLaunch];
-            for (int i = 0; i < nf; ++i) bucket[i] = zdirect_bucket(sep_radius(fb.f[i].kspec));
             ColDesc zd = z;
             zd.scale = (double)p.pz;
-            for (int rm : {0, 8, 12, 16}) {
-                FieldBatch sub;
-                memset(&sub, 0, sizeof sub);
-                int n = 0;
-                for (int i = 0; i < nf; ++i)
-                    if (bucket[i] == rm) sub.f[n++] = fb.f[i];
-                if (n == 0) continue;
-                const hipError_t e = rm ? launch_zdirect(rm, sub, n, zd, p.py, st)
-                                        : launch_col<T>(p.qz, sub, n, z, p.py, p.tw_z, st, p.var[VFD_PASS_ZCONV]);
-                if (e != hipSuccess) return e;
-            }
-            return hipSuccess;
+            for (int rm : {8, 12, 16})
+                if (hipError_t e = launch_zdirect(rm, fb, nf, zd, p.py, st)) return e;
         }
     }
     return launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[VFD_PASS_ZCONV]);
@@ -316,7 +285,8 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             }
         }
         GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, p.pad_mode, p.kspec_format == VFD_KSPEC_COMPLEX_STD,
-                   p.field_bytes, p.work_bytes, p.kspec_bytes, p.field_bytes};
+                   p.field_bytes, p.work_bytes, p.kspec_bytes, p.field_bytes,
+                   p.kspec_format == VFD_KSPEC_COMPLEX_STD ? 1.0 : (double)p.px * p.py * p.pz};
         ColDesc y{};
         y.mode = COL_FWD; y.rs = p.rs_y; y.nin = p.ny; y.nout = p.py; y.ncols = p.hx;
         y.rstride = p.ld; y.pstride = p.plane; y.scale = 1.0;
@@ -414,6 +384,15 @@ int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms) {
     if (int rc = hip_check(hipMalloc((void**)&buf, total), "autotune scratch")) return rc;
     hipLaunchKernelGGL(scratch_fill_kernel, dim3(2048), dim3(256), 0, st, (unsigned*)buf, total / 4, 0x5EEDu);
     int rc = hip_check(hipGetLastError(), "autotune fill");
+    if (!rc && p.kspec_format == VFD_KSPEC_SEPARABLE) {
+        // a real separable table, so that the z pass runs the form its radius
+        // selects: the schedule variants only matter for the FFT form, which
+        // FP32 tables take beyond radius kZDirectMaxR, so the window is made
+        // wider than that where the zero padding allows
+        const int rmax = std::min({p.px - p.nx, p.py - p.ny, p.pz - p.nz});
+        const int R = std::min(rmax, kZDirectMaxR + 3);
+        if (R >= 1) rc = plan_gauss_spectrum(p, ((R - 0.5) / 6.0) * ((R - 0.5) / 6.0), 1.0, buf + fbytes, st);
+    }
     std::vector<vfd_field> fl(nfields);
     for (int i = 0; i < nfields; ++i)
         fl[i] = vfd_field{buf, buf, buf + fbytes + p.kspec_bytes + (size_t)i * p.work_bytes, buf + fbytes, 0.0, 0, 0};
@@ -466,7 +445,7 @@ static int run_r2c(const Plan& p, const void* in, void* out, hipStream_t st) {
     fb.f[0].in = in;
     fb.f[0].work = out;
     const size_t cbytes = (size_t)p.pz * p.py * p.hx * 2 * p.prec;
-    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, p.field_bytes, cbytes, 0, 0};
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, p.field_bytes, cbytes, 0, 0, 1.0};
     hipError_t e = launch_xfwd<T>(p.L, fb, 1, g, p.tw_x, st);
     if (e != hipSuccess) return hip_check(e, "r2c x launch");
     ColDesc y{};
@@ -498,7 +477,7 @@ static int run_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
     y.nin = p.py; y.nout = p.py; y.rstride = p.ld; y.pstride = p.plane;
     e = launch_col<T>(p.py, fb, 1, y, p.nz, p.tw_y, st);
     if (e != hipSuccess) return hip_check(e, "c2r y launch");
-    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, 0, cbytes, 0, p.field_bytes};
+    GridDesc g{p.nx, p.ny, p.nz, p.L, p.ld, p.plane, VFD_PAD_ZERO, 0, 0, cbytes, 0, p.field_bytes, 1.0};
     e = launch_xinv<T>(p.L, fb, 1, g, p.tw_x, st);
     return e == hipSuccess ? VFD_OK : hip_check(e, "c2r x launch");
 }
@@ -589,7 +568,6 @@ int plan_gauss_spectrum(const Plan& p, double sigma2, double decay, void* d_kspe
                            "gauss tables upload");
         // the host vector must outlive the async copy
         if (!rc) rc = hip_check(hipStreamSynchronize(st), "gauss tables sync");
-        if (!rc) sep_radius_set(d_kspec, R);
         return rc;
     }
     double* d = nullptr;

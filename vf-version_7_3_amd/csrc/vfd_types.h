@@ -42,6 +42,10 @@ struct GridDesc {
     int pad_mode;       // PadMode
     int kspec_complex;  // 1: complex spectrum, 0: real
     unsigned long in_bytes, work_bytes, kspec_bytes, out_bytes;  // per-field array sizes
+    // constant-pad restore term base * K^(0) * kdc_scale: the complex spectrum
+    // is stored unnormalized (1), the real one with 1/(Px Py Pz) folded in
+    // (Px Py Pz)
+    double kdc_scale;
 };
 
 // One column pass (y or z).
