@@ -170,6 +170,17 @@ int vfd_diffuse_update(const vfd_plan* plan, const vfd_field* fields, float* con
                        const float* d_frag_col, const float* d_frag_ela, float* const* d_grads, double* totals,
                        void* stream);
 
+/* vfd_diffuse_update without any device allocation, for callers that cannot
+ * allocate (code under hipGraph capture, tight serving loops).  No counterpart
+ * in the reference.  Same kernels and bits as vfd_diffuse_update; the partial
+ * sums of the totals go to a fixed library-owned device array (128 KiB + 64 B
+ * per GPU, held until the process ends; any voxel count fits), and the 8
+ * totals to d_totals, device memory (optional).  Asynchronous: never
+ * synchronizes.  Calls that overlap on one GPU share that array and race. */
+int vfd_diffuse_update_noalloc(const vfd_plan* plan, const vfd_field* fields, float* const* d_delta,
+                               const float* decay, const float* d_frag_col, const float* d_frag_ela,
+                               float* const* d_grads, double* d_totals, void* stream);
+
 /* Visualisation downsample: sampleChem (src/Visualization/3D/volumeRender_kernel.cu:
  * 215-322) into a dense array instead of a surface.  d_dst[dz][dy][dx] =
  * min(d_src[stride*dz][stride*dy][stride*dx] * scale[chem_index], 1) for

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/vfd.h"
 #include "vfd_internal.h"
@@ -177,18 +178,32 @@ extern "C" int vfd_sample_chem(const float* d_src, int D, int H, int W, int chem
     return hip_check(hipGetLastError(), "sample_chem launch");
 }
 
-// levels (unless already formed), gradients and totals for 8 chems of n voxels
+// Device bytes chem_pass needs for its partial sums: one (sum, compensation)
+// pair per chem and work-group (at most kBlocks work-groups, whatever n is),
+// then the 8 totals.
+constexpr size_t kPassScratchBytes = (size_t)kChems * kBlocks * sizeof(double2) + kChems * sizeof(double);
+
+// levels (unless already formed), gradients and totals for 8 chems of n voxels.
+// buf: kPassScratchBytes of device memory for the partial sums, or null to
+// allocate them on the stream; d_totals: a device destination of the 8 totals
+// instead of the host array `totals` (both may be null).
 static int chem_pass(const ChemPtrs& cp, bool levels_in_p, const float* d_frag_col, const float* d_frag_ela,
-                     float* const* d_grads, long n, double* totals, hipStream_t st) {
+                     float* const* d_grads, long n, double* totals, hipStream_t st, void* buf = nullptr,
+                     double* d_totals = nullptr) {
     const int nb = (int)std::min<long>(kBlocks, std::max<long>(1, (n + kThreads - 1) / kThreads));
     const long chunk = (n + nb - 1) / nb;
     double2* partial = nullptr;
     double* d_tot = nullptr;
     const size_t pbytes = (size_t)kChems * nb * sizeof(double2);
-    if (int rc = hip_check(hipMallocAsync((void**)&partial, pbytes + kChems * sizeof(double), st),
-                           "hipMallocAsync(chem update)"))
+    if (buf) {
+        if (pbytes + kChems * sizeof(double) > kPassScratchBytes)
+            return set_error(VFD_ERR_INVALID, "chem update: %d partial sums exceed the scratch array", kChems * nb);
+        partial = (double2*)buf;
+    } else if (int rc = hip_check(hipMallocAsync((void**)&partial, pbytes + kChems * sizeof(double), st),
+                                  "hipMallocAsync(chem update)")) {
         return rc;
-    d_tot = (double*)((char*)partial + pbytes);
+    }
+    d_tot = d_totals ? d_totals : (double*)((char*)partial + pbytes);
     if (levels_in_p)
         hipLaunchKernelGGL(chem_update_kernel<true>, dim3(nb), dim3(kThreads), 0, st, cp, d_frag_col, d_frag_ela,
                            d_grads[0], d_grads[1], d_grads[2], n, chunk, partial);
@@ -205,8 +220,37 @@ static int chem_pass(const ChemPtrs& cp, bool levels_in_p, const float* d_frag_c
                        "chem totals download");
         if (!rc) rc = hip_check(hipStreamSynchronize(st), "chem totals sync");
     }
-    (void)hipFreeAsync(partial, st);
+    if (!buf) (void)hipFreeAsync(partial, st);
     return rc;
+}
+
+// ------------------------------------------------ allocation-free variant
+// The partial sums of vfd_diffuse_update_noalloc live in this file-scope
+// array, which each GPU the library runs on holds until the process ends.  In
+// host code `scratch` names a host-side stand-in: kernels get the device
+// address of each GPU's copy, looked up once per device (scratch_addr).
+static __device__ __align__(256) unsigned char scratch[kPassScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kMaxDevices];
+
+// device address of the current device's `scratch`, looked up on the first
+// call on that device with the device current
+static int scratch_addr(void** out) {
+    int dev = 0;
+    if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return set_error(VFD_ERR_UNSUPPORTED, "scratch: device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        if (int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+            return rc;
+        if (!p) return set_error(VFD_ERR_DEVICE, "scratch: null device address");
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return VFD_OK;
 }
 
 namespace vfd {
@@ -257,4 +301,35 @@ extern "C" int vfd_chem_update(const float* const* d_t, float* const* d_d, float
     for (int g = 0; g < 3; ++g)
         if (!d_grads[g]) return set_error(VFD_ERR_INVALID, "chem_update: gradient %d is null", g);
     return chem_pass(cp, false, d_frag_col, d_frag_ela, d_grads, n, totals, (hipStream_t)stream);
+}
+
+// vfd_diffuse_update without any device allocation: the same kernels (and so
+// the same bits), with the partial sums of the totals in `scratch` and the
+// totals written to device memory.  Calls that overlap on one GPU share its
+// `scratch` and race; nothing in it needs clearing (every partial sum a call
+// reads, that call has written).
+extern "C" int vfd_diffuse_update_noalloc(const vfd_plan* plan, const vfd_field* fields, float* const* d_delta,
+                                          const float* decay, const float* d_frag_col, const float* d_frag_ela,
+                                          float* const* d_grads, double* d_totals, void* stream) {
+    if (!plan || !fields || !d_delta || !decay || !d_frag_col || !d_frag_ela || !d_grads)
+        return set_error(VFD_ERR_INVALID, "diffuse_update_noalloc: null argument");
+    const Plan& p = plan->p;
+    if (p.prec != VFD_F32) return set_error(VFD_ERR_UNSUPPORTED, "diffuse_update_noalloc: FP32 plans only");
+    ChemPtrs cp{};
+    for (int ic = 0; ic < kChems; ++ic) {
+        if (!d_delta[ic]) return set_error(VFD_ERR_INVALID, "diffuse_update_noalloc: chem %d has no delta field", ic);
+        if (!fields[ic].out) return set_error(VFD_ERR_INVALID, "diffuse_update_noalloc: chem %d has no output", ic);
+        cp.p[ic] = (float*)fields[ic].out;
+    }
+    for (int g = 0; g < 3; ++g)
+        if (!d_grads[g]) return set_error(VFD_ERR_INVALID, "diffuse_update_noalloc: gradient %d is null", g);
+    void* buf = nullptr;
+    if (int rc = scratch_addr(&buf)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    Epilogue ep;
+    ep.deltas = d_delta;
+    ep.decay = decay;
+    if (int rc = plan_diffuse_passes(p, fields, kChems, 0, 5, st, &ep)) return rc;
+    const long n = (long)p.nx * p.ny * p.nz;
+    return chem_pass(cp, true, d_frag_col, d_frag_ela, d_grads, n, nullptr, st, buf, d_totals);
 }

@@ -55,3 +55,8 @@ int plan_r2c(const Plan& p, const void* in, void* out, hipStream_t st);
 int plan_c2r(const Plan& p, void* in, void* out, hipStream_t st);
 
 }  // namespace vfd
+
+// the opaque plan handle of the C-ABI (include/vfd.h)
+struct vfd_plan {
+    vfd::Plan p;
+};

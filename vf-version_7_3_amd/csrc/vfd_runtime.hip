@@ -707,9 +707,6 @@ bool length_supported(int q) {
 // ====================================================================== C-ABI
 using namespace vfd;
 
-struct vfd_plan {
-    Plan p;
-};
 
 extern "C" {
 

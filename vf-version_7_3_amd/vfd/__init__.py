@@ -72,7 +72,9 @@ lib.vfd_field_mass.argtypes = [_p, ctypes.POINTER(_p), _i, ctypes.POINTER(_d), _
 lib.vfd_r2c_3d.argtypes = [_p, _p, _p, _p]
 lib.vfd_diffuse_update.argtypes = [_p, ctypes.POINTER(Field), ctypes.POINTER(_p), ctypes.POINTER(ctypes.c_float),
                                    _p, _p, ctypes.POINTER(_p), ctypes.POINTER(_d), _p]
-lib.vfd_diffuse_sample.argtypes = [_p, ctypes.POINTER(Field), _i, ctypes.POINTER(_i), _i, ctypes.POINTER(_p), _p]
+lib.vfd_diffuse_update_noalloc.argtypes = [_p, ctypes.POINTER(Field), ctypes.POINTER(_p),
+                                           ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), _p, _p]
+lib.vfd_diffuse_sample.argtypes =[_p, ctypes.POINTER(Field), _i, ctypes.POINTER(_i), _i, ctypes.POINTER(_p), _p]
 lib.vfd_sample_chem.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
 lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.POINTER(_p),
                                 ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
@@ -184,6 +186,20 @@ class Plan:
         check(lib.vfd_diffuse_update(self.h, arr, dv, dec, _ptr(frag_col), _ptr(frag_ela), gv,
                                      out if totals else None, _stream(stream)))
         return list(out) if totals else None
+
+    def diffuse_update_noalloc(self, fields, deltas, decay, frag_col, frag_ela, grads, d_totals=None, stream=None):
+        """diffuse_update without device allocation: the 8 totals go to the
+        FP64 device tensor d_totals (8 elements, optional); asynchronous."""
+        arr = fields if isinstance(fields, ctypes.Array) else self.field_array(fields)
+        if len(arr) != 8 or len(deltas) != 8 or len(decay) != 8 or len(grads) != 3:
+            raise ValueError("diffuse_update_noalloc: 8 fields, 8 deltas, 8 decays and 3 gradient fields")
+        if d_totals is not None and (d_totals.numel() < 8 or d_totals.element_size() != 8 or not d_totals.is_cuda):
+            raise ValueError("diffuse_update_noalloc: d_totals must be a device tensor of 8 FP64 values")
+        dv = (ctypes.c_void_p * 8)(*[x.data_ptr() for x in deltas])
+        gv = (ctypes.c_void_p * 3)(*[x.data_ptr() for x in grads])
+        dec = (ctypes.c_float * 8)(*[float(x) for x in decay])
+        check(lib.vfd_diffuse_update_noalloc(self.h, arr, dv, dec, _ptr(frag_col), _ptr(frag_ela), gv,
+                                             _ptr(d_totals), _stream(stream)))
 
     def diffuse_sample(self, fields, chem_index, stride, vis, stream=None):
         """diffuse() with the visualisation downsample of each field's output
