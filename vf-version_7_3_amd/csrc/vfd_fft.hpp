@@ -290,7 +290,26 @@ template <typename T, int Q, int TPF, bool REV, int MAXR = 16> struct FftShape {
     }
 };
 
-template <typename T, int Q, int TPF, bool INV, bool REV, int S, int NS, int MAXR> struct StageRunner {
+// Synchronization between the stages of one transform.  When the TPF threads
+// of every transform lie inside one wavefront (TPF divides 64) and each
+// transform has LDS of its own, the exchange involves that wave alone: its
+// LDS operations execute in issue order, so a compiler-level barrier at
+// wavefront scope is enough and the other waves of the work-group never wait
+// for it.  Otherwise the whole work-group synchronizes.
+constexpr int kWave = 64;
+template <int TPF> constexpr bool wave_local() { return TPF <= kWave && kWave % TPF == 0; }
+template <bool WAVE> __device__ __forceinline__ void stage_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+template <typename T, int Q, int TPF, bool INV, bool REV, int S, int NS, int MAXR, bool WAVE = false>
+struct StageRunner {
     static constexpr RadixPlan PL = make_plan(Q, REV, MAXR);
     static constexpr int NST = PL.n;
     static constexpr int R = PL.r[S];
@@ -343,21 +362,22 @@ template <typename T, int Q, int TPF, bool INV, bool REV, int S, int NS, int MAX
         if (FROM_LDS) load(t, v, lds);
         compute(t, v, tw, tws);
         if constexpr (S + 1 < NST) {
-            __syncthreads();  // every thread finished reading this stage's inputs
+            stage_sync<WAVE>();  // every thread finished reading this stage's inputs
             store(t, v, lds);
-            __syncthreads();  // stage outputs visible
-            StageRunner<T, Q, TPF, INV, REV, S + 1, NS * R, MAXR>::template run<true>(t, v, lds, tw, tws);
+            stage_sync<WAVE>();  // stage outputs visible
+            StageRunner<T, Q, TPF, INV, REV, S + 1, NS * R, MAXR, WAVE>::template run<true>(t, v, lds, tw, tws);
         }
     }
 };
 
 // Full length-Q transform of one transform per thread group.  On entry v holds
 // the first-stage inputs (FftShape<..,REV>::in_elem); on exit v holds the
-// last-stage outputs (FftShape<..,REV>::out_elem).  Contains __syncthreads():
-// every thread of the work-group must call it.
-template <typename T, int Q, int TPF, bool INV, bool REV, int MAXR = 16, typename LDS>
+// last-stage outputs (FftShape<..,REV>::out_elem).  Contains __syncthreads()
+// (every thread of the work-group must call it) unless WAVE, where each
+// transform's threads and LDS belong to one wavefront (stage_sync).
+template <typename T, int Q, int TPF, bool INV, bool REV, int MAXR = 16, bool WAVE = false, typename LDS>
 __device__ __forceinline__ void fft_regs(int t, cx<T>* v, LDS& lds, const cx<T>* tw, int tws) {
-    StageRunner<T, Q, TPF, INV, REV, 0, 1, MAXR>::template run<false>(t, v, lds, tw, tws);
+    StageRunner<T, Q, TPF, INV, REV, 0, 1, MAXR, WAVE>::template run<false>(t, v, lds, tw, tws);
 }
 
 // ------------------------------------------------------- buffer memory ops

@@ -49,7 +49,11 @@ static hipError_t row_go(const FieldBatch& fields, int nfields, const GridDesc& 
     const long rows = (long)g.Ny * g.Nz;
     const long items = (rows + C::G - 1) / C::G * nfields;
     auto kern = FWD ? xfwd_kernel<T, Q, V> : xinv_kernel<T, Q, V, VIS>;
-    const long cap = (long)num_cus() * blocks_per_cu(kern, C::NT);
+    // 4 rounds of resident work-groups: the rows whose waves drift apart
+    // (wave-level stage sync) balance better over the CUs than with one round
+    // (x passes 1.34 -> 1.31 ms at 1024x512x128, 8 fields), while a work-group
+    // per row group restages the twiddle table for every 4 rows (1.65 ms)
+    const long cap = 4L * num_cus() * blocks_per_cu(kern, C::NT);
     const unsigned grid = (unsigned)(items < cap ? items : cap);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(C::NT), 0, st, fields, g, (const cx<T>*)tw, nfields);
     return hipGetLastError();

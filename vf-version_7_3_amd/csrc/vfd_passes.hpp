@@ -36,6 +36,9 @@ template <typename T, int Q, int V = 0> struct RowCfg {
     static constexpr int G = (NTT / TPF) > 0 ? (NTT / TPF) : 1;     // rows per group
     static constexpr int NT = TPF * G;
     static constexpr int PITCH = Q + Q / 32 + 1;                 // padded LDS row (RowLds)
+    // each row's transform and LDS row belong to one wavefront: its stages
+    // synchronize that wave alone (stage_sync)
+    static constexpr bool WS = wave_local<TPF>();
     // twiddle table W_Px (2Q entries) staged in LDS when it fits beside the rows
     static constexpr bool TW_LDS = (2 * Q + G * PITCH) * (int)sizeof(cx<T>) <= 64 * 1024;
     static constexpr int TW_N = TW_LDS ? 2 * Q : 1;
@@ -129,18 +132,18 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 cx<T> val;
                 if (vec) {
                     const bool ok = n >= 0 && row_in && x1 < g.Nx;
-                    val = B::ld_cx(rin, ok ? rowoff + (O)x0 * RB : B::OOB);
+                    val = B::template ld_cx<kStream>(rin, ok ? rowoff + (O)x0 * RB : B::OOB);
                 } else {
                     const bool ok0 = n >= 0 && row_in && x0 >= xlo && x0 < xhi;
                     const bool ok1 = n >= 0 && row_in && x1 >= xlo && x1 < xhi;
-                    const T a0 = B::ld_r(rin, ok0 ? rowoff + (O)x0 * RB : B::OOB);
-                    const T a1 = B::ld_r(rin, ok1 ? rowoff + (O)x1 * RB : B::OOB);
+                    const T a0 = B::template ld_r<kStream>(rin, ok0 ? rowoff + (O)x0 * RB : B::OOB);
+                    const T a1 = B::template ld_r<kStream>(rin, ok1 ? rowoff + (O)x1 * RB : B::OOB);
                     val = cx<T>{ok0 ? a0 - base : T(0), ok1 ? a1 - base : T(0)};
                 }
                 v[b * Sh::R0 + r] = val;
             }
-        fft_regs<T, L, C::TPF, false, false, C::MAXR>(t, v, lds, tw, 2);
-        __syncthreads();
+        fft_regs<T, L, C::TPF, false, false, C::MAXR, C::WS>(t, v, lds, tw, 2);
+        stage_sync<C::WS>();
 #pragma unroll
         for (int b = 0; b < Sh::NBL; ++b)
 #pragma unroll
@@ -148,7 +151,7 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 const int k = Sh::out_elem(t, b, r);
                 if (k >= 0) lds.at(k) = v[b * Sh::RL + r];
             }
-        __syncthreads();
+        stage_sync<C::WS>();
         const O dst = (O)((long)z * g.plane + (long)y * g.ld) * CB;
         for (int k = t; k <= L; k += C::TPF) {
             const cx<T> zk = lds.at(k == L ? 0 : k);
@@ -220,7 +223,7 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 const cx<T> xo = cmulc(csub(ak, ac), tw[ok ? k : 0]);  // (A - A*) W^-k
                 v[b * Sh::R0 + r] = cx<T>{xe.x - xo.y, xe.y + xo.x};    // Xe + i Xo
             }
-        fft_regs<T, L, C::TPF, true, false, C::MAXR>(t, v, lds, tw, 2);
+        fft_regs<T, L, C::TPF, true, false, C::MAXR, C::WS>(t, v, lds, tw, 2);
         T addc = T(0);
         if (g.pad_mode == PAD_CONSTANT_VF) {
             const auto rk = B::make(fd.kspec, g.kspec_bytes);
@@ -413,7 +416,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 for (int r = 0; r < ShI::R0; ++r) {
                     const int m = ShI::in_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    x[s * NII + b * ShI::R0 + r] = B::ld_cx(rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
+                    x[s * NII + b * ShI::R0 + r] = B::template ld_cx<NTP>(rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
                 }
     }
     if constexpr (MODE != COL_FWD) {
