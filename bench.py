@@ -9,11 +9,17 @@ plus 50 point sources of amplitude U[1,100].  Inputs are synthetic and
 generated on the device; kernel spectra are precomputed and resident.
 
 A step = one tick of all fields (5 HBM passes per field, all fields batched
-into each launch).  W untimed warmup steps, then K timed steps bracketed by
-torch.cuda.synchronize().  Per-pass durations come from HIP events recorded
-between the launches on the launch stream over the timed region.  The CPU
-baseline is the oracle's FP32 OpenMP port timed on a bounded sample (one field
-of the same workload, one tick) on the host cores.
+into each launch; the z pass is four launches: the direct form for window
+radius bounds 8, 12, 16 and the FFT form, each acting on its own fields).  W
+untimed warmup steps, then K timed steps bracketed by torch.cuda.synchronize().
+Per-kernel durations come from HIP events recorded between the launches on the
+launch stream over the timed region; roofline.kernel is the single kernel with
+the largest average duration, roofline.step_frac the whole step.  copy_gbs is
+the library's 16-byte non-temporal streaming copy measured in the same run;
+fft_z_step repeats the step with the spectrum streamed as the real 3D array
+(the FFT form of the z pass for every field).  The CPU baseline is the
+oracle's FP32 OpenMP port timed on a bounded sample (five fields of the same
+workload, one tick each) on the host cores.
 
 Other workloads (one JSON line each, same fields where they apply):
   --workload ensemble  64 resident runs of uneven grids drawn from
@@ -69,6 +75,8 @@ def parse():
     ap.add_argument("--fields", type=int, default=8)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fft-z", action="store_true",
+                    help="step: skip the second measurement with the real (FFT z pass) spectrum")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default (or VFD_SCHEDULE) schedule instead of autotuning")
     ap.add_argument("--kspec", default="separable", choices=["separable", "real"],
@@ -114,27 +122,37 @@ def _header(args, metric, value, unit, ms_per_step, config, dtype="f32"):
 
 
 def bench_copy(args):
-    """Device-to-device copy of 1 GiB (torch): the measured HBM bandwidth that
-    sits beside the 8 TB/s spec peak, and the known-size kernel the HBM
-    counters (FETCH_SIZE, WRITE_SIZE) are calibrated on."""
+    """Device-to-device copy of 1 GiB with the library's streaming kernel
+    (16-byte non-temporal loads and stores, one per lane): the measured HBM
+    bandwidth that sits beside the 8 TB/s spec peak, and the known-size kernel
+    the HBM counters (FETCH_SIZE, WRITE_SIZE) are calibrated on.  torch's
+    copy_ of the same bytes is reported beside it."""
     import torch
+    import vfd
     n = (1 << 30) // 4
     src = torch.rand(n, device="cuda:0")
     dst = torch.empty_like(src)
-    for _ in range(args.warmup):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(args.steps):
-        dst.copy_(src)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / args.steps
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    ms = timed(lambda: vfd.stream_copy(dst, src, stream))
+    ms_torch = timed(lambda: dst.copy_(src))
     gbs = 2.0 * n * 4 / (ms * 1e-3) / 1e9
     out = _header(args, "HBM copy bandwidth (1 GiB device-to-device, read + write bytes)", gbs, "GB/s", ms,
-                  {"workload": "torch copy_ of a 1 GiB FP32 tensor"})
+                  {"workload": "vfd_stream_copy of 1 GiB (16-byte non-temporal loads/stores, one per lane)"})
     out["frac_of_spec"] = gbs / HBM_PEAK_GBS
+    out["torch_copy_gbs"] = 2.0 * n * 4 / (ms_torch * 1e-3) / 1e9
     print(json.dumps(out), flush=True)
 
 
@@ -618,6 +636,55 @@ def measured_traffic(shape, nfields, kspec, pass_name):
     return None if b is None else float(b["read"] + b["write"])
 
 
+def copy_bandwidth(vfd, torch, stream, nbytes=1 << 30, reps=7):
+    """GB/s of the library's 16-byte non-temporal streaming copy (read + write
+    bytes over time), median of reps, measured on `stream` with HIP events."""
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    vfd.stream_copy(b, a, stream)
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        vfd.stream_copy(b, a, stream)
+        e1.record(stream)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    del a, b
+    return 2.0 * nbytes / (ts[len(ts) // 2] * 1e-3) / 1e9
+
+
+def fft_z_step(vfd, torch, np, shape, fields, params, model_bytes, stream, warmup, steps):
+    """The same step with the spectrum streamed as the 3D real array: every
+    field takes the FFT form of the z pass (the form a non-separable kernel,
+    such as the ABM's stencil heat kernel, needs).  Fields are copies."""
+    plan = vfd.Plan(shape, kspec_format=vfd.KSPEC_REAL)
+    fs = [f.clone() for f in fields]
+    works = [plan.new_work() for _ in fs]
+    ks = [plan.gauss_spectrum(s2, dec, stream=stream) for (s2, dec) in params]
+    arr = plan.field_array([{"in": f, "out": f, "work": w, "kspec": k} for f, w, k in zip(fs, works, ks)])
+    schedule = plan.autotune(len(fs), stream)[0]
+    for _ in range(warmup):
+        plan.diffuse(arr, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.diffuse(arr, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nvox = float(np.prod(shape)) * len(fs)
+    out = {"kernel_spectrum": "real", "steps": steps, "ms_per_step": ms, "voxel_updates_s": nvox / (ms * 1e-3),
+           "bytes_per_step": model_bytes, "frac": model_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "schedule": dict(zip(vfd.PASS_NAMES, schedule))}
+    del arr, fs, works, ks
+    torch.cuda.empty_cache()
+    return out
+
+
 def synth_fields(shape, nfields, seed, device):
     import numpy as np
     import torch
@@ -724,33 +791,51 @@ def main():
         plan.diffuse(arr, stream)
     torch.cuda.synchronize()
 
+    # every launch of a step on its own, in the order vfd_diffuse issues them
+    # (the z pass is its four launches), with a HIP event between launches
+    launches = [("xfwd", vfd.PASS_XFWD), ("yfwd", vfd.PASS_YFWD)] + \
+               [(n, vfd.PASS_Z_DIRECT8 + i) for i, n in enumerate(vfd.Z_PART_NAMES)] + \
+               [("yinv", vfd.PASS_YINV), ("xinv", vfd.PASS_XINV)]
+    nl = len(launches)
     K = args.steps
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(K)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(nl + 1)] for _ in range(K)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(K):
         evs[k][0].record(stream)
-        for p in range(5):
+        for i, (_, p) in enumerate(launches):
             plan.diffuse_pass(arr, p, stream)
-            evs[k][p + 1].record(stream)
+            evs[k][i + 1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
-    pass_ms = [0.0] * 5
+    kern_ms = [0.0] * nl
     step_ms = []
     for k in range(K):
-        for p in range(5):
-            pass_ms[p] += evs[k][p].elapsed_time(evs[k][p + 1])
-        step_ms.append(evs[k][0].elapsed_time(evs[k][5]))
-    pass_ms = [t / K for t in pass_ms]
+        for i in range(nl):
+            kern_ms[i] += evs[k][i].elapsed_time(evs[k][i + 1])
+        step_ms.append(evs[k][0].elapsed_time(evs[k][nl]))
+    kern_ms = [t / K for t in kern_ms]
     nf = len(fields)
     ms_per_step = wall * 1e3 / K
     vox = float(nz * ny * nx) * nf
     value = vox * K / wall
-    pass_gbs = [plan.pass_bytes(p) * nf / (pass_ms[p] * 1e-3) / 1e9 for p in range(5)]
-    dom = int(np.argmax(pass_ms))
+    # fields per launch: the z launches act on the fields of their own form
+    # (window radius R = ceil(6 sigma) <= 8, <= 12, <= 16 direct; beyond, FFT)
+    radii = [int(np.ceil(6.0 * np.sqrt(s2))) for s2, _ in params]
+    if kfmt == vfd.KSPEC_SEPARABLE:
+        forms = [0 if r <= 8 else 1 if r <= 12 else 2 if r <= 16 else 3 for r in radii]
+    else:
+        forms = [3] * nf
+    kern_fields = [nf, nf] + [forms.count(i) for i in range(4)] + [nf, nf]
+    kern_bytes = [plan.pass_bytes(p) * kern_fields[i] for i, (_, p) in enumerate(launches)]
+    kern_gbs = [b / (t * 1e-3) / 1e9 if t > 0 else 0.0 for b, t in zip(kern_bytes, kern_ms)]
+    dom = int(np.argmax(kern_ms))
     step_bytes = plan.info.bytes_per_field_tick * nf
     mean_step_ms = float(np.mean(step_ms))
+    copy_gbs = copy_bandwidth(vfd, torch, stream)
+    fft_z = None if args.no_fft_z else fft_z_step(vfd, torch, np, shape, fields, params, model_bytes, stream,
+                                                  args.warmup, min(K, 20))
 
     out = {
         "metric": "voxel-updates/s for 8-field step at 1024x512x128 (% HBM roofline); p50/p99 latency",
@@ -775,20 +860,30 @@ def main():
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "roofline": {
             "bound": "hbm",
-            "kernel": vfd.PASS_NAMES[dom],
-            "achieved": pass_gbs[dom],
+            # the single kernel with the largest average duration
+            "kernel": launches[dom][0],
+            "achieved": kern_gbs[dom],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": pass_gbs[dom] / HBM_PEAK_GBS,
-            "traffic": measured_traffic(shape, nf, args.kspec, vfd.PASS_NAMES[dom]),
+            "frac": kern_gbs[dom] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(shape, nf, args.kspec, launches[dom][0]),
+            # the whole step: the bytes its launches move / its duration
             "step_frac": step_bytes / (mean_step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "passes_ms": dict(zip(vfd.PASS_NAMES, pass_ms)),
-            "passes_gbs": dict(zip(vfd.PASS_NAMES, pass_gbs)),
+            "kernels_ms": {n: t for (n, _), t in zip(launches, kern_ms)},
+            "kernels_gbs": {n: g for (n, _), g in zip(launches, kern_gbs)},
+            "kernel_fields": {n: c for (n, _), c in zip(launches, kern_fields)},
             "algorithmic_bytes_per_step": step_bytes,
+            # measured streaming reference: 16-byte non-temporal copy of 1 GiB
+            "copy_gbs": copy_gbs,
+            "step_frac_of_copy": step_bytes / (mean_step_ms * 1e-3) / 1e9 / copy_gbs,
+            # the survey's byte model with a streamed real K^ (8N + 32 A1 + 32 A2 + 4 A3):
+            # counts 4 A3 per field that the separable spectrum never reads
             "model_bytes_per_step": model_bytes,
             "model_frac": model_bytes / (mean_step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
     }
+    if fft_z is not None:
+        out["fft_z_step"] = fft_z
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(shape, args.seed)
     print(json.dumps(out), flush=True)

@@ -112,8 +112,19 @@ int vfd_diffuse(const vfd_plan* plan, const vfd_field* fields, int nfields, void
  * for per-pass timing.  vfd_plan_pass_bytes gives the algorithmic HBM bytes
  * of one pass for one field (sum over passes = bytes_per_field_tick). */
 enum { VFD_PASS_XFWD = 0, VFD_PASS_YFWD = 1, VFD_PASS_ZCONV = 2, VFD_PASS_YINV = 3, VFD_PASS_XINV = 4 };
+/* The single launches of the z pass, for per-kernel timing: the direct form
+ * of radius bound 8, 12 and 16 (FP32 separable plans; no launch otherwise) and
+ * the FFT form.  Each covers every field and acts on the fields of its own
+ * form, so issuing all four equals VFD_PASS_ZCONV.  vfd_plan_pass_bytes gives
+ * the bytes of one field of that form. */
+enum { VFD_PASS_Z_DIRECT8 = 5, VFD_PASS_Z_DIRECT12 = 6, VFD_PASS_Z_DIRECT16 = 7, VFD_PASS_Z_FFT = 8 };
 int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream);
 double vfd_plan_pass_bytes(const vfd_plan* plan, int pass);
+
+/* Device-to-device copy of `bytes` (a multiple of 16, below 4 GiB) with one
+ * 16-byte non-temporal load and store per lane: the streaming-bandwidth
+ * reference the passes are measured against.  Asynchronous. */
+int vfd_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 /* Schedule autotuner: every compiled schedule variant (radix 16/8, work-group
  * size) of every pass is timed on scratch buffers for nfields fields; the

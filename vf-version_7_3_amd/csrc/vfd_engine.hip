@@ -443,8 +443,8 @@ int vfd_run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id) {
         s.plan = plan;
         // schedule for this shape, tuned once on scratch buffers when they fit
         const int nf = std::min(8, d->nfields);
-        if (!getenv("VFD_NO_AUTOTUNE") && e->used + (size_t)nf * plan.work_bytes + plan.field_bytes +
-                                                  plan.kspec_bytes <= e->budget)
+        if (!getenv("VFD_NO_AUTOTUNE") &&
+            e->used + (size_t)nf * (plan.work_bytes + plan.field_bytes + plan.kspec_bytes) <= e->budget)
             (void)plan_autotune(s.plan, nf, nullptr, nullptr);
         sit = e->shapes.emplace(key, s).first;
         Held h;

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/vfd.h"
+#include "vfd_fft.hpp"
 #include "vfd_internal.h"
 #include "vfd_launch.h"
 
@@ -245,23 +246,29 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
 // launches (radius bounds 8, 12, 16) and the FFT launch each cover every field
 // of the batch, and a work-group whose field is not of its form exits at once
 // (zform_of in vfd_passes.hpp), so no host-side record of the tables is needed.
+// part selects one launch (0, 1, 2: the direct form of radius bound 8, 12,
+// 16; 3: the FFT form) for per-kernel timing, -1 all of them.
 template <typename T>
-static hipError_t launch_z(const Plan& p, const FieldBatch& fb, int nf, const ColDesc& z, hipStream_t st) {
+static hipError_t launch_z(const Plan& p, const FieldBatch& fb, int nf, const ColDesc& z, hipStream_t st,
+                           int part = -1) {
     if constexpr (sizeof(T) == 4) {
         if (p.kspec_format == VFD_KSPEC_SEPARABLE) {
             ColDesc zd = z;
             zd.scale = (double)p.pz;
-            for (int rm : {8, 12, 16})
-                if (hipError_t e = launch_zdirect(rm, fb, nf, zd, p.py, st)) return e;
+            const int rms[3] = {8, 12, 16};
+            for (int i = 0; i < 3; ++i)
+                if (part < 0 || part == i)
+                    if (hipError_t e = launch_zdirect(rms[i], fb, nf, zd, p.py, st)) return e;
         }
     }
+    if (part >= 0 && part != 3) return hipSuccess;
     return launch_col<T>(p.qz, fb, nf, z, p.py, p.tw_z, st, p.var[VFD_PASS_ZCONV]);
 }
 
 // ------------------------------------------------------------- execution
 template <typename T>
 static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pass_lo, int pass_hi,
-                    hipStream_t st, const Epilogue* epi) {
+                    hipStream_t st, const Epilogue* epi, int zpart = -1) {
     for (int f0 = 0; f0 < nfields; f0 += kMaxFieldsPerLaunch) {
         const int nf = std::min(kMaxFieldsPerLaunch, nfields - f0);
         FieldBatch fb;
@@ -309,7 +316,7 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
             switch (pass) {
                 case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
                 case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st, p.var[pass]); break;
-                case VFD_PASS_ZCONV: e = launch_z<T>(p, fb, nf, z, st); break;
+                case VFD_PASS_ZCONV: e = launch_z<T>(p, fb, nf, z, st, zpart); break;
                 case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
                 case VFD_PASS_XINV:
                     e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass], epi && epi->vis);
@@ -338,6 +345,14 @@ int plan_diffuse_passes(const Plan& p, const vfd_field* fields, int nfields, int
                              : run_tick<double>(p, fields, nfields, lo, hi, st, epi);
 }
 
+// one launch of the z pass (part 0..3, see launch_z)
+static int plan_diffuse_zpart(const Plan& p, const vfd_field* fields, int nfields, int part, hipStream_t st) {
+    if (int rc = check_fields(fields, nfields)) return rc;
+    if (nfields == 0) return VFD_OK;
+    return p.prec == VFD_F32 ? run_tick<float>(p, fields, nfields, 2, 3, st, nullptr, part)
+                             : run_tick<double>(p, fields, nfields, 2, 3, st, nullptr, part);
+}
+
 double plan_pass_bytes(const Plan& p, int pass) {
     const double N = (double)p.nx * p.ny * p.nz;
     const double A1 = (double)p.hx * p.ny * p.nz, A2 = (double)p.hx * p.py * p.nz,
@@ -349,7 +364,11 @@ double plan_pass_bytes(const Plan& p, int pass) {
     switch (pass) {
         case VFD_PASS_XFWD: return r * N + c * A1;
         case VFD_PASS_YFWD: return c * A1 + c * A2;
-        case VFD_PASS_ZCONV: return 2 * c * A2 + kbytes;
+        case VFD_PASS_ZCONV:
+        case VFD_PASS_Z_DIRECT8:
+        case VFD_PASS_Z_DIRECT12:
+        case VFD_PASS_Z_DIRECT16:
+        case VFD_PASS_Z_FFT: return 2 * c * A2 + kbytes;  // per field of the launch's form
         case VFD_PASS_YINV: return c * A2 + c * A1;
         case VFD_PASS_XINV: return c * A1 + r * N;
         default: return 0.0;
@@ -374,50 +393,68 @@ __global__ void scratch_fill_kernel(unsigned* __restrict__ p, size_t nwords, uns
 
 // Autotuner: time every compiled schedule variant of every pass on scratch
 // buffers of nfields fields filled with non-zero data, keep the fastest per
-// pass (median of 5 timed launches), and remember it for this shape.
+// pass (median of 5 timed launches within steps), and remember it for this
+// shape.  Every
+// scratch field has its own field, spectrum and work arrays, as in a real
+// step: fields sharing one input would be read from the last-level cache and
+// time the row passes at a bandwidth no real step sees.
 int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms) {
     if (nfields < 1) nfields = 1;
     if (nfields > kMaxFieldsPerLaunch) nfields = kMaxFieldsPerLaunch;
     const size_t fbytes = (size_t)p.nx * p.ny * p.nz * p.prec;
-    const size_t total = fbytes + p.kspec_bytes + (size_t)nfields * p.work_bytes;
+    const size_t per_field = fbytes + p.kspec_bytes + p.work_bytes;  // each a multiple of 8
+    const size_t total = (size_t)nfields * per_field;
     char* buf = nullptr;
     if (int rc = hip_check(hipMalloc((void**)&buf, total), "autotune scratch")) return rc;
     hipLaunchKernelGGL(scratch_fill_kernel, dim3(2048), dim3(256), 0, st, (unsigned*)buf, total / 4, 0x5EEDu);
     int rc = hip_check(hipGetLastError(), "autotune fill");
+    std::vector<vfd_field> fl(nfields);
+    for (int i = 0; i < nfields; ++i) {
+        char* f = buf + (size_t)i * per_field;
+        fl[i] = vfd_field{f, f, f + fbytes + p.kspec_bytes, f + fbytes, 0.0, 0, 0};
+    }
     if (!rc && p.kspec_format == VFD_KSPEC_SEPARABLE) {
-        // a real separable table, so that the z pass runs the form its radius
+        // real separable tables, so that the z pass runs the form their radius
         // selects: the schedule variants only matter for the FFT form, which
         // FP32 tables take beyond radius kZDirectMaxR, so the window is made
         // wider than that where the zero padding allows
         const int rmax = std::min({p.px - p.nx, p.py - p.ny, p.pz - p.nz});
         const int R = std::min(rmax, kZDirectMaxR + 3);
-        if (R >= 1) rc = plan_gauss_spectrum(p, ((R - 0.5) / 6.0) * ((R - 0.5) / 6.0), 1.0, buf + fbytes, st);
+        for (int i = 0; i < nfields && !rc && R >= 1; ++i)
+            rc = plan_gauss_spectrum(p, ((R - 0.5) / 6.0) * ((R - 0.5) / 6.0), 1.0, (void*)fl[i].kspec, st);
     }
-    std::vector<vfd_field> fl(nfields);
-    for (int i = 0; i < nfields; ++i)
-        fl[i] = vfd_field{buf, buf, buf + fbytes + p.kspec_bytes + (size_t)i * p.work_bytes, buf + fbytes, 0.0, 0, 0};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc) rc = hip_check(hipEventCreate(&e0), "event");
     if (!rc) rc = hip_check(hipEventCreate(&e1), "event");
     const int saved[5] = {p.var[0], p.var[1], p.var[2], p.var[3], p.var[4]};
+    // a variant is timed inside whole steps (the pass between events, the
+    // other passes around it): a pass repeated on its own meets the caches in
+    // a state no step leaves them in (x forward: 1.54 ms for the variant
+    // chosen that way against 1.31 ms for the best one within steps)
+    auto step_timed = [&](int pass, float* ms) {
+        int rc2 = VFD_OK;
+        for (int q = 0; q < 5 && !rc2; ++q) {
+            if (q == pass) rc2 = hip_check(hipEventRecord(e0, st), "event");
+            if (!rc2) rc2 = plan_diffuse_passes(p, fl.data(), nfields, q, q + 1, st);
+            if (!rc2 && q == pass) rc2 = hip_check(hipEventRecord(e1, st), "event");
+        }
+        if (!rc2 && ms) rc2 = hip_check(hipEventSynchronize(e1), "event sync");
+        if (!rc2 && ms) rc2 = hip_check(hipEventElapsedTime(ms, e0, e1), "event time");
+        return rc2;
+    };
     for (int pass = 0; pass < 5 && !rc; ++pass) {
         float best = 1e30f;
         int bestv = saved[pass];
         for (int v = 0; v < kScheduleVariants && !rc; ++v) {
             p.var[pass] = v;
-            rc = plan_diffuse_passes(p, fl.data(), nfields, pass, pass + 1, st);  // warm-up
-            float t[5];
-            for (int k = 0; k < 5 && !rc; ++k) {
-                rc = hip_check(hipEventRecord(e0, st), "event");
-                if (!rc) rc = plan_diffuse_passes(p, fl.data(), nfields, pass, pass + 1, st);
-                if (!rc) rc = hip_check(hipEventRecord(e1, st), "event");
-                if (!rc) rc = hip_check(hipEventSynchronize(e1), "event sync");
-                if (!rc) rc = hip_check(hipEventElapsedTime(&t[k], e0, e1), "event time");
-            }
+            rc = step_timed(pass, nullptr);  // warm-up
+            constexpr int kReps = 5;
+            float t[kReps];
+            for (int k = 0; k < kReps && !rc; ++k) rc = step_timed(pass, &t[k]);
             if (rc) break;
-            std::sort(t, t + 5);
-            if (t[2] < best) {
-                best = t[2];
+            std::sort(t, t + kReps);
+            if (t[kReps / 2] < best) {
+                best = t[kReps / 2];
                 bestv = v;
             }
         }
@@ -686,6 +723,18 @@ int plan_c2r(const Plan& p, void* in, void* out, hipStream_t st) {
     return p.prec == VFD_F32 ? run_c2r<float>(p, in, out, st) : run_c2r<double>(p, in, out, st);
 }
 
+// The HBM streaming reference: one 16-byte element per lane over a full grid,
+// non-temporal loads and stores (measured on MI355X: this form streams at
+// about 6.4 TB/s where grid-stride loops and 8-byte lanes stay near 5.5).
+__global__ void __launch_bounds__(256) stream_copy_kernel(const void* src, void* dst, unsigned nvec) {
+    const rsrc_t rs = make_rsrc(src, (unsigned long)nvec * 16u);
+    const rsrc_t rd = make_rsrc(dst, (unsigned long)nvec * 16u);
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    const unsigned off = i < nvec ? i * 16u : kBufOOB;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, kStream), rd, off, 0,
+                                           kStream);
+}
+
 bool length_supported(int q) {
     switch (q) {
 #define VFD_CASE(Q) case Q:
@@ -783,7 +832,20 @@ int vfd_diffuse_sample(const vfd_plan* plan, const vfd_field* fields, int nfield
 
 int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields, int pass, void* stream) {
     if (!plan) return set_error(VFD_ERR_INVALID, "diffuse_pass: null plan");
+    if (pass >= VFD_PASS_Z_DIRECT8 && pass <= VFD_PASS_Z_FFT)
+        return plan_diffuse_zpart(plan->p, fields, nfields, pass - VFD_PASS_Z_DIRECT8, (hipStream_t)stream);
     return plan_diffuse_passes(plan->p, fields, nfields, pass, pass + 1, (hipStream_t)stream);
+}
+
+int vfd_stream_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return set_error(VFD_ERR_INVALID, "stream_copy: null buffer");
+    if (bytes % 16 || bytes >= kBufLimit)
+        return set_error(VFD_ERR_INVALID, "stream_copy: %zu bytes (a multiple of 16 below 4 GiB)", bytes);
+    if (bytes == 0) return VFD_OK;
+    const unsigned nvec = (unsigned)(bytes / 16);
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((nvec + 255) / 256), dim3(256), 0, (hipStream_t)stream, src, dst,
+                       nvec);
+    return hip_check(hipGetLastError(), "stream_copy launch");
 }
 
 double vfd_plan_pass_bytes(const vfd_plan* plan, int pass) { return plan ? plan_pass_bytes(plan->p, pass) : 0.0; }

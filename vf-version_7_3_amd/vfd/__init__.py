@@ -20,6 +20,9 @@ PAD_CONSTANT_VF, PAD_ZERO = 3, 100
 KSPEC_REAL, KSPEC_COMPLEX_STD, KSPEC_SEPARABLE = 0, 1, 2
 PASS_XFWD, PASS_YFWD, PASS_ZCONV, PASS_YINV, PASS_XINV = 0, 1, 2, 3, 4
 PASS_NAMES = ["xfwd", "yfwd", "zconv", "yinv", "xinv"]
+# the single launches of the z pass (per-kernel timing); together they are PASS_ZCONV
+PASS_Z_DIRECT8, PASS_Z_DIRECT12, PASS_Z_DIRECT16, PASS_Z_FFT = 5, 6, 7, 8
+Z_PART_NAMES = ["z_direct8", "z_direct12", "z_direct16", "z_fft"]
 
 
 class VfdError(RuntimeError):
@@ -80,6 +83,16 @@ lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.P
                                 ctypes.POINTER(ctypes.c_float), _p, _p, ctypes.POINTER(_p), ctypes.c_long,
                                 ctypes.POINTER(_d), _p]
 lib.vfd_c2r_3d.argtypes = [_p, _p, _p, _p]
+lib.vfd_stream_copy.argtypes = [_p, _p, _sz, _p]
+
+
+def stream_copy(dst, src, stream=None):
+    """Copy src to dst (device tensors of equal byte size, a multiple of 16)
+    with the library's 16-byte non-temporal streaming kernel."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("stream_copy: tensors differ in size")
+    check(lib.vfd_stream_copy(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
 
 
 def check(rc):
