@@ -29,10 +29,14 @@ WHChemical::~WHChemical() {
 // info[0..12] = kernel FFT box, kernel window, kernelRadius, niter, chem FFT
 // box (D,H,W), chem window (KD,KH,KW), centre (KZ,KY,KX).  Between ticks the
 // diffused field becomes the next input (the ABM's update with no agents and
-// no extra decay).  spectra (optional) receives the 8 chem spectra.
+// no extra decay).  spectra (optional) receives the 8 chem spectra; windows
+// (optional) the 8 kernel-window slots (windowD x windowH x windowW each; the
+// kernel context holds one window buffer per GPU, which every chem's build
+// reuses, DiffusionHelper.cpp:166-240); rows (optional) the first spectrum
+// row (kz = ky = 0, FFTW/2 + 1 values) of every chem.
 extern "C" int diffuser_check(int nx, int ny, int nz, float dt, const float* lambda, const float* gamma,
                               const float* baseline, int rightWall, int ticks, float* fields, fComplex* spectra,
-                              int* info) {
+                              float* windows, fComplex* rows, int* info) {
     c_ctx* kc = new c_ctx();
     c_ctx* cc = new c_ctx();
     std::memset(kc, 0, sizeof(c_ctx));
@@ -59,6 +63,12 @@ extern "C" int diffuser_check(int nx, int ny, int nz, float dt, const float* lam
         const size_t sn = (size_t)cc->FFTD * cc->FFTH * (cc->FFTW / 2 + 1);
         for (int ic = 0; ic < 8 && spectra && !rc; ++ic)
             rc = vfdMemcpy(spectra + ic * sn, cc->d_kernelspectrum_h[ic], sn * sizeof(fComplex));
+        const size_t wn = (size_t)kc->windowD * kc->windowH * kc->windowW;
+        for (int ic = 0; ic < 8 && windows && !rc; ++ic)
+            rc = vfdMemcpy(windows + ic * wn, kc->d_data[ic], wn * sizeof(float));
+        const size_t rn = (size_t)cc->FFTW / 2 + 1;
+        for (int ic = 0; ic < 8 && rows && !rc; ++ic)
+            rc = vfdMemcpy(rows + ic * rn, cc->d_kernelspectrum_h[ic], rn * sizeof(fComplex));
         Diffuser d(8, cc, kc, &chem);
         for (int t = 0; t < ticks && !rc; ++t) {
             d.diffuseChemGPU(rightWall);

@@ -28,7 +28,7 @@ def check_lib(vfd):
     lib = ctypes.CDLL(CHECK)
     fp = ctypes.POINTER(ctypes.c_float)
     lib.diffuser_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, fp, fp, fp,
-                                   ctypes.c_int, ctypes.c_int, fp, fp, ctypes.POINTER(ctypes.c_int)]
+                                   ctypes.c_int, ctypes.c_int, fp, fp, fp, fp, ctypes.POINTER(ctypes.c_int)]
     return lib
 
 
@@ -53,7 +53,7 @@ def test_diffuser_ticks_match_oracle(check_lib, orc, shape, dt, ticks):
     cap = 8 * 128 * 128 * 65
     spectra = np.zeros(2 * cap, dtype=np.float32)
     rc = check_lib.diffuser_check(nx, ny, nz, dt, _fp(lam), _fp(gam), _fp(base), wall, ticks, _fp(fields),
-                                  _fp(spectra), info)
+                                  _fp(spectra), None, None, info)
     assert rc == 0
     kfft, kwin, krad, niter, FD, FH, FW, KD, KH, KW, KZ, KY, KX = list(info)
     print(f"kernel box {kfft} window {kwin} radius {krad} niter {niter}; chem box {FD}x{FH}x{FW} "
