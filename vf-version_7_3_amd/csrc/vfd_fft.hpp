@@ -67,6 +67,66 @@ constexpr double cos_(double x) {
     }
     return s;
 }
+
+// FP32 rotation constants.  A rotation w = c - i s rounded componentwise to
+// FP32 has |w|^2 = c^2 + s^2 != 1: applied on every pass, its magnitude error
+// is a gain of the frequencies whose path crosses it, the same every tick, so
+// it accumulates linearly over ticks (with the plain roundings W16^1 has
+// |w|^2 - 1 = -5.7e-8 and W8^1 -3.4e-8; a radix-16 stage then loses about
+// 1.1e-8 of amplitude on average, about 9e-8 per tick over the x and y
+// passes).  Phase errors cancel between a forward rotation and the inverse
+// one (its conjugate); magnitude errors do not.  So each pair is picked among
+// the FP32 neighbours (+-2 ulp) of the rounded (c, s) for c^2 + s^2 closest
+// to a target magnitude, the smaller phase error breaking ties.
+struct Rot {
+    float c, s;
+};
+constexpr double ulp_at(float x) {  // spacing of the floats just above |x|
+    double a = x < 0 ? -(double)x : (double)x;
+    if (a == 0.0) return 1.4e-45;
+    double p = 1.0;
+    while (p > a) p *= 0.5;
+    while (p * 2.0 <= a) p *= 2.0;
+    return p * (1.0 / 8388608.0);  // 2^-23
+}
+constexpr float step_float(float x, int k) {  // k floats away from x (k in -2..2)
+    float y = x;
+    for (int i = 0; i < (k < 0 ? -k : k); ++i) {
+        const double u = ulp_at(y);
+        // below a power of two the spacing halves: try the half step first
+        const float half = (float)((double)y + (k < 0 ? -0.5 : 0.5) * u);
+        y = half != y ? half : (float)((double)y + (k < 0 ? -u : u));
+    }
+    return y;
+}
+constexpr double absd(double x) { return x < 0 ? -x : x; }
+constexpr Rot best_rot(double c, double s, double target = 1.0) {
+    const float c0 = (float)c, s0 = (float)s;
+    Rot best{c0, s0};
+    double be = 1e300, ba = 1e300;
+    for (int i = -2; i <= 2; ++i)
+        for (int j = -2; j <= 2; ++j) {
+            const float cc = step_float(c0, i), ss = step_float(s0, j);
+            const double e = absd((double)cc * cc + (double)ss * ss - target);
+            const double a = absd((double)cc - c) + absd((double)ss - s);
+            if (e < be || (e == be && a < ba)) {
+                best = Rot{cc, ss};
+                be = e;
+                ba = a;
+            }
+        }
+    return best;
+}
+constexpr double mag2(Rot r) { return (double)r.c * r.c + (double)r.s * r.s; }
+// W16^1 (both directions) and the radix-3 pair (-1/2, sqrt(3)/2)
+constexpr Rot kW16 = best_rot(0.92387953251128675613, 0.38268343236508977173);
+constexpr Rot kW3 = best_rot(-0.5, 0.86602540378443864676);
+// W8^1 = (1 - i)/sqrt(2): near 45 degrees the FP32 lattice moves |w|^2 in
+// steps of about 8e-8, so the forward pair stays the plain rounding and the
+// inverse pair is picked for |w_fwd|^2 |w_inv|^2 closest to 1 (a round trip
+// of 1 + 1.6e-8 instead of 1 - 6.8e-8)
+constexpr Rot kW8F = Rot{(float)0.70710678118654752440, (float)0.70710678118654752440};
+constexpr Rot kW8I = best_rot(0.70710678118654752440, 0.70710678118654752440, 1.0 / mag2(kW8F));
 }  // namespace ce
 
 // ------------------------------------------------------------ butterflies
@@ -92,15 +152,31 @@ template <bool INV, typename T> struct Bfly<4, INV, T> {
     }
 };
 
-// multiply by W8^1 = h(1 - i) (fwd) / h(1 + i) (inv)
+// The rotation pairs (c, s) of the butterflies: FP32 takes the magnitude-picked
+// pairs of ce (forward and inverse may differ so that their magnitudes cancel
+// over a round trip), FP64 the correctly rounded values.
+template <typename T, bool INV> struct Rots {
+    static constexpr T w8c = T(0.70710678118654752440), w8s = T(0.70710678118654752440);
+    static constexpr T w16c = T(0.92387953251128675613), w16s = T(0.38268343236508977173);
+    static constexpr T w3c = T(-0.5), w3s = T(0.86602540378443864676);
+};
+template <bool INV> struct Rots<float, INV> {
+    static constexpr ce::Rot w8 = INV ? ce::kW8I : ce::kW8F;
+    static constexpr ce::Rot w16 = INV ? ce::best_rot(ce::kW16.c, ce::kW16.s, 1.0 / ce::mag2(ce::kW16)) : ce::kW16;
+    static constexpr ce::Rot w3 = INV ? ce::best_rot(ce::kW3.c, ce::kW3.s, 1.0 / ce::mag2(ce::kW3)) : ce::kW3;
+    static constexpr float w8c = w8.c, w8s = w8.s, w16c = w16.c, w16s = w16.s, w3c = w3.c, w3s = w3.s;
+};
+
+// multiply by W8^1 = (1 - i)/sqrt(2) (fwd) / (1 + i)/sqrt(2) (inv), as the
+// pair (a, b) ~ (1/sqrt(2), 1/sqrt(2)): c (a -+ i b)
 template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_1(cx<T> c) {
-    const T h = T(0.70710678118654752440084436210484903928);
-    return INV ? cx<T>{h * (c.x - c.y), h * (c.x + c.y)} : cx<T>{h * (c.x + c.y), h * (c.y - c.x)};
+    constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
+    return INV ? cx<T>{c.x * a - c.y * b, c.y * a + c.x * b} : cx<T>{c.x * a + c.y * b, c.y * a - c.x * b};
 }
-// multiply by W8^3 = h(-1 - i) (fwd) / h(-1 + i) (inv)
+// multiply by W8^3 = (-1 - i)/sqrt(2) (fwd) / (-1 + i)/sqrt(2) (inv): c (-a -+ i b)
 template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_3(cx<T> c) {
-    const T h = T(0.70710678118654752440084436210484903928);
-    return INV ? cx<T>{-h * (c.x + c.y), h * (c.x - c.y)} : cx<T>{h * (c.y - c.x), -h * (c.x + c.y)};
+    constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
+    return INV ? cx<T>{-c.x * a - c.y * b, c.x * b - c.y * a} : cx<T>{c.y * b - c.x * a, -c.y * a - c.x * b};
 }
 
 template <bool INV, typename T> struct Bfly<8, INV, T> {
@@ -125,8 +201,8 @@ template <bool INV, typename T> struct Bfly<8, INV, T> {
 // 16-point DFT as 4 x 4: x[k1 + 4 k2]; DFT4 over k2, twiddle W16^{k1 m2}, DFT4 over k1.
 template <bool INV, typename T> struct Bfly<16, INV, T> {
     static __device__ __forceinline__ void run(cx<T>* v) {
-        const T c1 = T(0.92387953251128675612818318939678828682);  // cos(pi/8)
-        const T s1 = T(0.38268343236508977172845998403039886676);  // sin(pi/8)
+        constexpr T c1 = Rots<T, INV>::w16c;  // cos(pi/8)
+        constexpr T s1 = Rots<T, INV>::w16s;  // sin(pi/8)
         cx<T> a[4][4];
 #pragma unroll
         for (int k1 = 0; k1 < 4; ++k1) {
@@ -163,11 +239,12 @@ template <bool INV, typename T> struct Bfly<16, INV, T> {
 
 template <bool INV, typename T> struct Bfly<3, INV, T> {
     static __device__ __forceinline__ void run(cx<T>* v) {
-        const T s3 = T(0.86602540378443864676372317075293618347);
+        constexpr T c3 = Rots<T, INV>::w3c;  // cos(2 pi/3) = -1/2
+        constexpr T s3 = Rots<T, INV>::w3s;  // sin(2 pi/3)
         cx<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
         cx<T> a0 = v[0];
         v[0] = cadd(a0, s);
-        cx<T> t = cx<T>{a0.x - T(0.5) * s.x, a0.y - T(0.5) * s.y};
+        cx<T> t = cx<T>{a0.x + c3 * s.x, a0.y + c3 * s.y};
         // fwd: y1 = t - i s3 d ; y2 = t + i s3 d
         cx<T> m = mul_mi<INV>(cscale(d, s3));
         v[1] = cadd(t, m);
@@ -175,9 +252,30 @@ template <bool INV, typename T> struct Bfly<3, INV, T> {
     }
 };
 
+// cos and sin of 2 pi q / R, q < R, as the rotation pairs of Rots (FP32:
+// magnitude-picked, the inverse pair compensating the forward one)
+template <typename T, bool INV, int R> struct OddTab {
+    T c[R], s[R];
+    constexpr OddTab() : c(), s() {
+        for (int q = 0; q < R; ++q) {
+            const double a = 2.0 * ce::PI * double(q) / double(R);
+            if constexpr (sizeof(T) == 4) {
+                const ce::Rot f = ce::best_rot(ce::cos_(a), ce::sin_(a));
+                const ce::Rot r = INV ? ce::best_rot(ce::cos_(a), ce::sin_(a), 1.0 / ce::mag2(f)) : f;
+                c[q] = r.c;
+                s[q] = r.s;
+            } else {
+                c[q] = T(ce::cos_(a));
+                s[q] = T(ce::sin_(a));
+            }
+        }
+    }
+};
+
 // Generic odd-prime DFT using the (n, R-n) symmetry; constants are compile time.
 template <int R, bool INV, typename T> struct BflyOdd {
     static __device__ __forceinline__ void run(cx<T>* v) {
+        constexpr OddTab<T, INV, R> tab{};
         constexpr int H = (R - 1) / 2;
         cx<T> a[H + 1], b[H + 1];
 #pragma unroll
@@ -196,8 +294,8 @@ template <int R, bool INV, typename T> struct BflyOdd {
             T cr = x0.x, ci = x0.y, sr = 0, si = 0;
 #pragma unroll
             for (int n = 1; n <= H; ++n) {
-                const T c = T(ce::cos_(2.0 * ce::PI * double((n * k) % R) / double(R)));
-                const T s = T(ce::sin_(2.0 * ce::PI * double((n * k) % R) / double(R)));
+                const T c = tab.c[(n * k) % R];
+                const T s = tab.s[(n * k) % R];
                 cr += a[n].x * c;
                 ci += a[n].y * c;
                 sr += b[n].x * s;
