@@ -241,6 +241,11 @@ void vfd_engine_destroy(vfd_engine* engine);
  * spectra; work buffers are pooled per shape and charged when grown). */
 size_t vfd_run_bytes(const vfd_run_desc* desc);
 int vfd_run_create(vfd_engine* engine, const vfd_run_desc* desc, int* run_id);
+/* vfd_run_create that waits for memory: when the run does not fit the budget,
+ * the call blocks until runs destroyed on other threads free enough of it,
+ * for at most timeout_s seconds (then VFD_ERR_BUDGET).  A run larger than
+ * the whole budget is refused at once. */
+int vfd_run_create_wait(vfd_engine* engine, const vfd_run_desc* desc, double timeout_s, int* run_id);
 int vfd_run_destroy(vfd_engine* engine, int run_id);
 /* device pointer of a resident field (row-major [nz][ny][nx], precision of the run) */
 int vfd_run_field_ptr(vfd_engine* engine, int run_id, int field, void** d_field);

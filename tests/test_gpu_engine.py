@@ -201,3 +201,39 @@ def test_run_lifecycle_errors(vfd, orc, eng_mod):
         assert used > 0
         eng.destroy_run(rid)
         assert eng.stats()["used_bytes"] == 0
+
+
+def test_admission_waits_for_memory(vfd, orc, eng_mod):
+    """vfd_run_create_wait: a run that does not fit waits until another run is
+    destroyed, then is admitted; it gives up with VFD_ERR_BUDGET at its
+    timeout; a run larger than the whole budget is refused at once."""
+    import threading
+    import time
+    print("case admission_waits_for_memory", flush=True)
+    shape, nf = (32, 64, 64), 4
+    run = eng_mod.run_bytes(shape, nf)
+    work = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE).info.work_bytes
+    s2, dc = [2.0] * nf, [0.99] * nf
+    with eng_mod.Engine(2 * run + work + run // 2) as eng:   # room for two runs and the shape's work buffer
+        a = eng.create_run(shape, s2, dc)
+        eng.create_run(shape, s2, dc)
+        with pytest.raises(vfd.VfdError) as ei:
+            eng.create_run(shape, s2, dc)                    # no wait: refused
+        assert ei.value.code == vfd.VFD_ERR_BUDGET
+        t0 = time.perf_counter()
+        with pytest.raises(vfd.VfdError) as ei:
+            eng.create_run_wait(shape, s2, dc, 0.3)          # waits, then gives up
+        assert ei.value.code == vfd.VFD_ERR_BUDGET and time.perf_counter() - t0 >= 0.25
+        with pytest.raises(vfd.VfdError) as ei:
+            eng.create_run_wait((64, 128, 128), [2.0] * 32, [0.99] * 32, 30.0)   # never fits
+        assert ei.value.code == vfd.VFD_ERR_BUDGET
+        got = []
+        th = threading.Thread(target=lambda: got.append(eng.create_run_wait(shape, s2, dc, 60.0)))
+        th.start()
+        time.sleep(0.5)
+        assert th.is_alive() and not got                     # still waiting for memory
+        eng.destroy_run(a)
+        th.join(30.0)
+        assert not th.is_alive() and len(got) == 1
+        st = eng.stats()
+        assert st["runs"] == 2 and st["used_bytes"] <= st["budget_bytes"]

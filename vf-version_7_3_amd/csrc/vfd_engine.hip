@@ -396,24 +396,37 @@ size_t vfd_run_bytes(const vfd_run_desc* d) {
     return run_bytes_for(p, d->nfields);
 }
 
-int vfd_run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id) {
+// Admit a run: wait_s <= 0 refuses at once when the run does not fit the
+// budget; otherwise the call waits up to wait_s seconds for destroyed runs to
+// free the memory (a run larger than the whole budget is refused at once).
+static int run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id, double wait_s) {
     if (!e || !d || !run_id || !d->sigma2 || !d->decay)
         return set_error(VFD_ERR_INVALID, "run_create: null argument");
     if (d->nfields < 1 || d->nfields > kMaxRunFields)
         return set_error(VFD_ERR_INVALID, "run_create: nfields %d outside 1..%d", d->nfields, kMaxRunFields);
-    std::lock_guard<std::mutex> lk(e->mu);
+    std::unique_lock<std::mutex> lk(e->mu);
     const ShapeKey key{d->nz, d->ny, d->nx, d->precision};
-    auto sit = e->shapes.find(key);
-    const bool new_shape = sit == e->shapes.end();
+    const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                             std::chrono::duration<double>(wait_s > 0 ? wait_s : 0.0));
+    std::map<ShapeKey, Shape>::iterator sit;
+    bool new_shape;
     Plan plan;
-    if (new_shape) {
-        if (int rc = make_plan_for(key, &plan)) return rc;
-    } else {
-        plan = sit->second.plan;
+    size_t need;
+    for (;;) {  // the shape may appear or go while this call waits
+        sit = e->shapes.find(key);
+        new_shape = sit == e->shapes.end();
+        if (new_shape) {
+            if (int rc = make_plan_for(key, &plan)) return rc;
+        } else {
+            plan = sit->second.plan;
+        }
+        need = run_bytes_for(plan, d->nfields) + (new_shape ? plan.work_bytes : 0);
+        if (e->used + need <= e->budget) break;
+        if (need > e->budget || wait_s <= 0 ||
+            (e->cv.wait_until(lk, deadline) == std::cv_status::timeout && e->used + need > e->budget))
+            return set_error(VFD_ERR_BUDGET, "run_create: %zu bytes needed, %zu of %zu in use", need, e->used,
+                             e->budget);
     }
-    const size_t need = run_bytes_for(plan, d->nfields) + (new_shape ? plan.work_bytes : 0);
-    if (e->used + need > e->budget)
-        return set_error(VFD_ERR_BUDGET, "run_create: %zu bytes needed, %zu of %zu in use", need, e->used, e->budget);
     Run r;
     r.key = key;
     r.nfields = d->nfields;
@@ -464,6 +477,12 @@ int vfd_run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id) {
     return VFD_OK;
 }
 
+int vfd_run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id) { return run_create(e, d, run_id, 0.0); }
+
+int vfd_run_create_wait(vfd_engine* e, const vfd_run_desc* d, double timeout_s, int* run_id) {
+    return run_create(e, d, run_id, timeout_s);
+}
+
 int vfd_run_destroy(vfd_engine* e, int run_id) {
     if (!e) return set_error(VFD_ERR_INVALID, "run_destroy: null engine");
     std::lock_guard<std::mutex> lk(e->mu);
@@ -491,6 +510,7 @@ int vfd_run_destroy(vfd_engine* e, int run_id) {
         }
     }
     e->runs.erase(it);
+    e->cv.notify_all();  // admissions waiting for memory (vfd_run_create_wait) recheck
     return VFD_OK;
 }
 

@@ -28,6 +28,7 @@ lib.vfd_engine_destroy.restype = None
 lib.vfd_run_bytes.argtypes = [ctypes.POINTER(RunDesc)]
 lib.vfd_run_bytes.restype = _sz
 lib.vfd_run_create.argtypes = [_p, ctypes.POINTER(RunDesc), ctypes.POINTER(_i)]
+lib.vfd_run_create_wait.argtypes = [_p, ctypes.POINTER(RunDesc), _d, ctypes.POINTER(_i)]
 lib.vfd_run_destroy.argtypes = [_p, _i]
 lib.vfd_run_field_ptr.argtypes = [_p, _i, _i, ctypes.POINTER(_p)]
 lib.vfd_run_write_field.argtypes = [_p, _i, _i, _p, _p]
@@ -85,6 +86,15 @@ class Engine:
         d = _desc(shape, sigma2, decay, precision)
         rid = _i()
         check(lib.vfd_run_create(self.h, ctypes.byref(d), ctypes.byref(rid)))
+        self._shape[rid.value] = (tuple(shape), len(sigma2), precision)
+        return rid.value
+
+    def create_run_wait(self, shape, sigma2, decay, timeout_s, precision=F32):
+        """create_run that waits up to timeout_s seconds for memory freed by
+        runs destroyed on other threads (ctypes releases the GIL meanwhile)."""
+        d = _desc(shape, sigma2, decay, precision)
+        rid = _i()
+        check(lib.vfd_run_create_wait(self.h, ctypes.byref(d), float(timeout_s), ctypes.byref(rid)))
         self._shape[rid.value] = (tuple(shape), len(sigma2), precision)
         return rid.value
 
