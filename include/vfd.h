@@ -82,7 +82,18 @@ typedef struct {
     size_t work_bytes;  /* per-field work buffer */
     size_t kspec_bytes; /* per-field kernel spectrum in the plan's format */
     double bytes_per_field_tick; /* algorithmic HBM bytes of one tick (pruned model) */
+    int order;          /* pass order, VFD_ORDER_XFIRST or VFD_ORDER_XLAST */
 } vfd_plan_info;
+
+/* Pass orders.  XFIRST: x R2C, y, z (fused forward * K^ * inverse), y inverse,
+ * x C2R.  XLAST: z R2C (two real columns per complex transform), y, x fused,
+ * y inverse, z C2R -- the same operator factored the other way, which keeps
+ * the x padding out of HBM; FP32 plans with VFD_KSPEC_COMPLEX_STD take it when
+ * it moves at most 0.9x the bytes of XFIRST (a box padded most along x, as the
+ * reference's 1152x1536x256 box of a 1006x1390x110 field) and both its
+ * lengths are compiled.  The environment variable VFD_ORDER=0 (1) at plan
+ * creation forces XFIRST (XLAST where possible). */
+enum { VFD_ORDER_XFIRST = 0, VFD_ORDER_XLAST = 1 };
 
 typedef struct {
     const void* in;    /* real field [nz][ny][nx] */
@@ -109,7 +120,8 @@ int vfd_diffuse(const vfd_plan* plan, const vfd_field* fields, int nfields, void
 
 /* The five passes of vfd_diffuse, one at a time (same arguments), in order
  * VFD_PASS_XFWD, VFD_PASS_YFWD, VFD_PASS_ZCONV, VFD_PASS_YINV, VFD_PASS_XINV;
- * for per-pass timing.  vfd_plan_pass_bytes gives the algorithmic HBM bytes
+ * for per-pass timing.  In the XLAST order the same five indices run z R2C,
+ * y forward, the fused x pass, y inverse and z C2R.  vfd_plan_pass_bytes gives the algorithmic HBM bytes
  * of one pass for one field (sum over passes = bytes_per_field_tick). */
 enum { VFD_PASS_XFWD = 0, VFD_PASS_YFWD = 1, VFD_PASS_ZCONV = 2, VFD_PASS_YINV = 3, VFD_PASS_XINV = 4 };
 /* The single launches of the z pass, for per-kernel timing: the direct form

@@ -517,6 +517,15 @@ template <> struct BufOps<float> {
     template <int A = kCacheDefault> static __device__ __forceinline__ void st_r(float v, H r, O off) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, A);
     }
+    // two adjacent complex elements (16 bytes, 16-byte aligned offset)
+    template <int A = kCacheDefault> static __device__ __forceinline__ void ld_cx2(H r, O off, float2& a, float2& b) {
+        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, A));
+        a = float2{q.x, q.y};
+        b = float2{q.z, q.w};
+    }
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx2(float2 a, float2 b, H r, O off) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, float4{a.x, a.y, b.x, b.y}), r, off, 0, A);
+    }
 };
 // FP64: global addresses with 64-bit byte offsets, so one array may exceed
 // 4 GiB (the FP64 work array of a 1536x768x192 grid is 7.4 GB).  The handle
@@ -560,6 +569,14 @@ template <> struct BufOps<double> {
         double* q = (double*)(h.p + off);
         if constexpr (A == kStream) __builtin_nontemporal_store(v, q);
         else *q = v;
+    }
+    template <int A = kCacheDefault> static __device__ __forceinline__ void ld_cx2(H h, O off, double2& a, double2& b) {
+        a = ld_cx<A>(h, off);
+        b = ld_cx<A>(h, off == OOB ? OOB : off + 16);
+    }
+    template <int A = kCacheDefault> static __device__ __forceinline__ void st_cx2(double2 a, double2 b, H h, O off) {
+        st_cx<A>(a, h, off);
+        st_cx<A>(b, h, off == OOB ? OOB : off + 16);
     }
 };
 

@@ -21,6 +21,12 @@ struct Plan {
     double bytes_per_field_tick;
     const void *tw_x, *tw_y, *tw_z;
     int var[5];  // schedule variant per pass
+    // pass order: VFD_ORDER_XFIRST (x R2C, y, z fused, y, x C2R) or
+    // VFD_ORDER_XLAST (z R2C, y, x fused, y, z C2R; FP32 complex spectrum)
+    int order;
+    int hz;        // x-last: pz/2 + 1 z half-spectrum planes
+    long ldx;      // x-last: work row stride (complex elements)
+    long xplane;   // x-last: work plane stride py * ldx
 };
 
 int set_error(int code, const char* fmt, ...);

@@ -29,4 +29,13 @@ hipError_t launch_col(int Q, const FieldBatch& fields, int nfields, const ColDes
 hipError_t launch_zdirect(int rm, const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                           hipStream_t st);
 
+// x-last order (FP32): z R2C (fwd) / C2R (inverse) of column pairs with FFT
+// length pz, and the fused x row pass of length px.  xlast_supported tells
+// whether both lengths are compiled.
+bool xlast_supported(int pz, int px);
+hipError_t launch_zr(bool fwd, int pz, const FieldBatch& fields, int nfields, const XlDesc& d, const void* tw,
+                     hipStream_t st, int variant = 0);
+hipError_t launch_xc(int px, const FieldBatch& fields, int nfields, const XlDesc& d, const void* tw,
+                     hipStream_t st, int variant = 0);
+
 }  // namespace vfd

@@ -387,6 +387,9 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     // narrower tile (long columns) shares each line with its neighbours, which
     // must still find it in the cache
     constexpr int NTP = C::TC * (int)CB >= 128 ? kStream : kCacheDefault;
+    // stores: non-temporal down to 64-byte row segments; a narrower segment is
+    // written back through the L2, where the tiles sharing its line merge
+    constexpr int NTS = C::TC * (int)CB >= 128 ? kStream : kCacheDefault;
     constexpr int NI = ShF::NB0 * ShF::R0;
     constexpr int NO = ShI::NBL * ShI::RL;
     constexpr int NK = ShF::NBL * ShF::RL;
@@ -530,7 +533,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 for (int r = 0; r < ShF::RL; ++r) {
                     const int m = ShF::out_elem(t, b, r);
                     const bool ok = m >= 0 && live;
-                    B::template st_cx<kStream>(v[b * ShF::RL + r], rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
+                    B::template st_cx<NTS>(v[b * ShF::RL + r], rwork, ok ? base + (O)(m * RS + s) * rstep : B::OOB);
                 }
             continue;
         } else {
@@ -557,7 +560,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
             for (int r = 0; r < ShI::RL; ++r) {
                 const int n = ShI::out_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nout && live;
-                B::template st_cx<kStream>(acc[b * ShI::RL + r], rwork, ok ? base + (O)n * rstep : B::OOB);
+                B::template st_cx<NTS>(acc[b * ShI::RL + r], rwork, ok ? base + (O)n * rstep : B::OOB);
             }
     }
 }

@@ -98,11 +98,11 @@ int twiddles(int p, int prec, const void** out) {
 // ------------------------------------------------------- schedule cache
 // Tuned per-pass schedule variants, keyed by device and plan shape.
 struct ShapeKey {
-    int dev, prec, pad, kfmt, nz, ny, nx, pz, py, px;
+    int dev, prec, pad, kfmt, nz, ny, nx, pz, py, px, order;
     bool operator<(const ShapeKey& o) const {
-        const int a[10] = {dev, prec, pad, kfmt, nz, ny, nx, pz, py, px};
-        const int b[10] = {o.dev, o.prec, o.pad, o.kfmt, o.nz, o.ny, o.nx, o.pz, o.py, o.px};
-        for (int i = 0; i < 10; ++i)
+        const int a[11] = {dev, prec, pad, kfmt, nz, ny, nx, pz, py, px, order};
+        const int b[11] = {o.dev, o.prec, o.pad, o.kfmt, o.nz, o.ny, o.nx, o.pz, o.py, o.px, o.order};
+        for (int i = 0; i < 11; ++i)
             if (a[i] != b[i]) return a[i] < b[i];
         return false;
     }
@@ -113,7 +113,7 @@ static std::map<ShapeKey, std::array<int, 5>> g_sched;
 static ShapeKey shape_key(const Plan& q) {
     int dev = 0;
     (void)hipGetDevice(&dev);
-    return ShapeKey{dev, q.prec, q.pad_mode, q.kspec_format, q.nz, q.ny, q.nx, q.pz, q.py, q.px};
+    return ShapeKey{dev, q.prec, q.pad_mode, q.kspec_format, q.nz, q.ny, q.nx, q.pz, q.py, q.px, q.order};
 }
 
 void tuned_lookup(Plan& q) {
@@ -229,6 +229,28 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
                           : q.kspec_format == VFD_KSPEC_COMPLEX_STD ? c * A3
                                                                      : (double)q.kspec_bytes;
     q.bytes_per_field_tick = 2 * r * N + 4 * c * A1 + 4 * c * A2 + kbytes;
+    // x-last order: z R2C over column pairs (hz planes of ny rows of 2 nxp
+    // columns), y to py rows, x fused with K^ read at every kx (px per row)
+    q.order = VFD_ORDER_XFIRST;
+    const char* so = getenv("VFD_ORDER");
+    const int want = so && *so ? atoi(so) : -1;
+    if (want != 0 && q.prec == VFD_F32 && q.kspec_format == VFD_KSPEC_COMPLEX_STD && !q.full_box &&
+        xlast_supported(q.pz, q.px)) {
+        const int hz = q.pz / 2 + 1;
+        const long ldx = round_up(q.nx, 16);  // even, > nx when nx is odd; rows on 128-byte lines
+        const double nxp2 = 2.0 * ((q.nx + 1) / 2);
+        const double B1 = (double)q.nx * q.ny * hz, B1p = nxp2 * q.ny * hz, B2 = (double)q.nx * q.py * hz;
+        const double xl = 2 * r * N + 2 * c * B1p + 2 * c * B1 + 4 * c * B2 + c * (double)hz * q.py * q.px;
+        const size_t wb = (size_t)hz * q.py * ldx * cxb;
+        if ((want == 1 || xl <= 0.9 * q.bytes_per_field_tick) && wb < kBufLimit) {
+            q.order = VFD_ORDER_XLAST;
+            q.hz = hz;
+            q.ldx = ldx;
+            q.xplane = (long)q.py * ldx;
+            q.work_bytes = wb;
+            q.bytes_per_field_tick = xl;
+        }
+    }
     // schedule variants per pass: tuned choice for this shape if known, else
     // VFD_SCHEDULE="x,y,z,yi,xi" or schedule 0
     for (int i = 0; i < 5; ++i) q.var[i] = 0;
@@ -272,6 +294,43 @@ static hipError_t launch_z(const Plan& p, const FieldBatch& fb, int nf, const Co
 }
 
 // ------------------------------------------------------------- execution
+// The x-last order's passes (FP32) over one launch's fields; the y passes are
+// the x-first order's column pass on the x-last work layout.
+static int run_xlast(const Plan& p, const FieldBatch& fb, int nf, int pass_lo, int pass_hi, hipStream_t st) {
+    XlDesc d{};
+    d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+    d.px = p.px; d.py = p.py; d.pz = p.pz;
+    d.hz = p.hz;
+    d.nxp = (p.nx + 1) / 2;
+    d.ncolp = (long)p.ny * d.nxp;
+    d.ldx = p.ldx;
+    d.plane = p.xplane;
+    d.ks_hx = p.hx;
+    d.pad_mode = p.pad_mode;
+    d.scale = 1.0 / ((double)p.px * p.py * p.pz);
+    d.in_bytes = d.out_bytes = p.field_bytes;
+    d.work_bytes = p.work_bytes;
+    d.kspec_bytes = p.kspec_bytes;
+    ColDesc y{};
+    y.mode = COL_FWD; y.rs = p.rs_y; y.nin = p.ny; y.nout = p.py; y.ncols = p.nx;
+    y.rstride = p.ldx; y.pstride = p.xplane; y.scale = 1.0;
+    y.work_bytes = p.work_bytes; y.kspec_bytes = p.kspec_bytes;
+    ColDesc yi = y;
+    yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
+    for (int pass = pass_lo; pass < pass_hi; ++pass) {
+        hipError_t e = hipSuccess;
+        switch (pass) {
+            case VFD_PASS_XFWD: e = launch_zr(true, p.pz, fb, nf, d, p.tw_z, st, p.var[pass]); break;
+            case VFD_PASS_YFWD: e = launch_col<float>(p.qy, fb, nf, y, p.hz, p.tw_y, st, p.var[pass]); break;
+            case VFD_PASS_ZCONV: e = launch_xc(p.px, fb, nf, d, p.tw_x, st, p.var[pass]); break;
+            case VFD_PASS_YINV: e = launch_col<float>(p.qy, fb, nf, yi, p.hz, p.tw_y, st, p.var[pass]); break;
+            case VFD_PASS_XINV: e = launch_zr(false, p.pz, fb, nf, d, p.tw_z, st, p.var[pass]); break;
+        }
+        if (e != hipSuccess) return hip_check(e, "x-last pass launch");
+    }
+    return VFD_OK;
+}
+
 template <typename T>
 static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pass_lo, int pass_hi,
                     hipStream_t st, const Epilogue* epi, int zpart = -1) {
@@ -317,6 +376,12 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
         z.work_bytes = p.work_bytes; z.kspec_bytes = p.kspec_bytes;
         ColDesc yi = y;
         yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
+        if constexpr (sizeof(T) == 4) {
+            if (p.order == VFD_ORDER_XLAST) {
+                if (int rc = run_xlast(p, fb, nf, pass_lo, pass_hi, st)) return rc;
+                continue;
+            }
+        }
         for (int pass = pass_lo; pass < pass_hi; ++pass) {
             hipError_t e = hipSuccess;
             switch (pass) {
@@ -361,6 +426,21 @@ static int plan_diffuse_zpart(const Plan& p, const vfd_field* fields, int nfield
 
 double plan_pass_bytes(const Plan& p, int pass) {
     const double N = (double)p.nx * p.ny * p.nz;
+    if (p.order == VFD_ORDER_XLAST) {
+        // z R2C writes both columns of every pair (2 nxp per row)
+        const double c = 2.0 * p.prec, r = (double)p.prec;
+        const double B1 = (double)p.nx * p.ny * p.hz, B1p = 2.0 * ((p.nx + 1) / 2) * p.ny * p.hz,
+                     B2 = (double)p.nx * p.py * p.hz, K = (double)p.hz * p.py * p.px;
+        switch (pass) {
+            case VFD_PASS_XFWD: return r * N + c * B1p;
+            case VFD_PASS_YFWD: return c * B1 + c * B2;
+            case VFD_PASS_ZCONV:
+            case VFD_PASS_Z_FFT: return 2 * c * B2 + c * K;
+            case VFD_PASS_YINV: return c * B2 + c * B1;
+            case VFD_PASS_XINV: return c * B1p + r * N;
+            default: return 0.0;
+        }
+    }
     const double A1 = (double)p.hx * p.ny * p.nz, A2 = (double)p.hx * p.py * p.nz,
                  A3 = (double)p.hx * p.py * p.pz;
     const double r = (double)p.prec, c = 2.0 * p.prec;
@@ -790,6 +870,7 @@ int vfd_plan_get_info(const vfd_plan* plan, vfd_plan_info* info) {
     info->work_bytes = p.work_bytes;
     info->kspec_bytes = p.kspec_bytes;
     info->bytes_per_field_tick = p.bytes_per_field_tick;
+    info->order = p.order;
     return VFD_OK;
 }
 

@@ -69,6 +69,25 @@ struct ColDesc {
     int sep_g;          // separable: table offset (elements) of [R | g_0 .. g_kZDirectMaxR]
 };
 
+// The x-last pass order (z R2C, y C2C, x fused, y inverse, z C2R) for boxes
+// padded most along x.  Work of one field: complex [hz][py][ldx], hz = pz/2+1
+// z half-spectrum planes of py rows; element (kz, y, x) at kz*plane + y*ldx + x.
+struct XlDesc {
+    int nx, ny, nz;      // data box
+    int px, py, pz;      // FFT box
+    int hz;              // pz / 2 + 1
+    int nxp;             // real column pairs per data row, (nx + 1) / 2
+    long ncolp;          // column pairs per field, ny * nxp
+    long ldx;            // work row stride (complex elements), even, > nx when nx is odd
+    long plane;          // work plane stride py * ldx
+    long ks_hx;          // spectrum row length px/2 + 1 (the R2C layout [pz][py][px/2+1])
+    int pad_mode;        // PadMode
+    double scale;        // applied to K^ products (1 / (px py pz))
+    unsigned long in_bytes, work_bytes, kspec_bytes, out_bytes;  // per-field array sizes
+    // grouped 1-D launch of narrow tiles (set by the launcher)
+    unsigned grid_fields, grid_segs;
+};
+
 // Largest window radius the direct z pass takes (separable spectrum, FP32);
 // the separable table carries the taps g_0 .. g_kZDirectMaxR.  Wider windows
 // (20, 24 taps a side) compiled to scratch-memory window accesses and were

@@ -20,6 +20,10 @@ PAD_CONSTANT_VF, PAD_ZERO = 3, 100
 KSPEC_REAL, KSPEC_COMPLEX_STD, KSPEC_SEPARABLE = 0, 1, 2
 PASS_XFWD, PASS_YFWD, PASS_ZCONV, PASS_YINV, PASS_XINV = 0, 1, 2, 3, 4
 PASS_NAMES = ["xfwd", "yfwd", "zconv", "yinv", "xinv"]
+# pass orders (vfd_plan_info.order); the x-last order runs the same five pass
+# indices as z R2C, y forward, x fused, y inverse, z C2R
+ORDER_XFIRST, ORDER_XLAST = 0, 1
+XLAST_PASS_NAMES = ["zr2c", "yfwd", "xconv", "yinv", "zc2r"]
 # the single launches of the z pass (per-kernel timing); together they are PASS_ZCONV
 PASS_Z_DIRECT8, PASS_Z_DIRECT12, PASS_Z_DIRECT16, PASS_Z_FFT = 5, 6, 7, 8
 Z_PART_NAMES = ["z_direct8", "z_direct12", "z_direct16", "z_fft"]
@@ -48,7 +52,7 @@ class PlanDesc(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("nz", _i), ("ny", _i), ("nx", _i), ("pz", _i), ("py", _i), ("px", _i),
                 ("L", _i), ("hx", _i), ("ld", ctypes.c_long), ("rs_y", _i), ("rs_z", _i),
-                ("work_bytes", _sz), ("kspec_bytes", _sz), ("bytes_per_field_tick", _d)]
+                ("work_bytes", _sz), ("kspec_bytes", _sz), ("bytes_per_field_tick", _d), ("order", _i)]
 
 
 class Field(ctypes.Structure):
