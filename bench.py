@@ -480,6 +480,15 @@ def bench_dropin(args):
     for n in names:
         out[flows[n][0]] = {"ms_per_tick": t[n] * 1e3, "voxel_updates_s": vox / t[n]}
     out["pcie_bytes_per_tick"] = 2.0 * vox * 4
+    # the plan fftDiffuse3D builds for this box: its pass order and algorithmic
+    # HBM bytes (every pass streams its arrays once)
+    plan = vfd.Plan((nz, ny, nx), (FD, FH, FW), vfd.F32, vfd.PAD_CONSTANT_VF, vfd.KSPEC_COMPLEX_STD)
+    hbm = nf * plan.info.bytes_per_field_tick
+    out["pass_order"] = "x-last" if plan.info.order == vfd.ORDER_XLAST else "x-first"
+    out["hbm_bytes_per_tick"] = hbm
+    if "resident" in t:
+        out["device_resident_fftDiffuse3D"]["hbm_gbs"] = hbm / t["resident"] / 1e9
+        out["device_resident_fftDiffuse3D"]["hbm_frac"] = hbm / t["resident"] / 1e9 / HBM_PEAK_GBS
     if "pipelined" in t and "serial" in t:
         out["speedup_vs_serial"] = t["serial"] / t["pipelined"]
     print(json.dumps(out), flush=True)

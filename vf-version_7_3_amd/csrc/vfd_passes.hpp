@@ -305,6 +305,16 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
     }
 }
 
+// v * k for a spectrum factor formed in FP64.  FP32 carries k as a pair of
+// floats, k ~ kh + kl, so that the factor's own rounding (up to 6e-8 of it,
+// the same on every tick) does not become a per-tick gain: the DC factor is
+// the decay of the mass, and a gain of 6e-8 a tick is 6e-6 after 100 ticks.
+__device__ __forceinline__ float2 kscale(float2 v, double k) {
+    const float kh = (float)k, kl = (float)(k - (double)kh);
+    return float2{fmaf(kh, v.x, kl * v.x), fmaf(kh, v.y, kl * v.y)};
+}
+__device__ __forceinline__ double2 kscale(double2 v, double k) { return double2{v.x * k, v.y * k}; }
+
 // ------------------------------------------------------------ column pass
 // One strip = TC adjacent columns of one plane; a column is a line along y
 // (row stride ld, planes z) or along z (row stride plane, planes ky).
@@ -508,8 +518,8 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
 #pragma unroll
                     for (int r = 0; r < ShF::RL; ++r) {
                         const int m = ShF::out_elem(t, b, r);
-                        const T k = (T)(ka * gzs[m >= 0 ? m * RS + s : 0]);
-                        v[b * ShF::RL + r] = cscale(v[b * ShF::RL + r], k);
+                        const double kd = ka * gzs[m >= 0 ? m * RS + s : 0];
+                        v[b * ShF::RL + r] = kscale(v[b * ShF::RL + r], kd);
                     }
             } else {
 #pragma unroll
@@ -607,7 +617,13 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
     const double sc = BD::ld_r(tk, OD(0));
     const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + blockIdx.y) * 8u);
     const double gxv = BD::ld_r(tk, live ? (OD)(1 + col) * 8u : BD::OOB);
-    const float ka = (float)(((gxv * gyv) * sc) * cd.scale);  // cd.scale = Pz here
+    // The taps as rounded to FP32 sum (exactly) to gs rather than 1: the column
+    // factor takes up the difference, so the window conserves mass
+    double gs = (double)g[0];
+#pragma unroll
+    for (int j = 1; j <= RM; ++j) gs += 2.0 * (double)g[j];
+    const double kad = (((gxv * gyv) * sc) * cd.scale) / gs;  // cd.scale = Pz here
+    const float ka = (float)kad, kl = (float)(kad - (double)ka);
     auto row = [&](int m) -> O { return (live && m >= 0 && m < nz) ? base + (O)m * rstep : B::OOB; };
     // w[(n - RM + k) mod U] holds x[n - RM + k]; x[-RM .. RM] to start
     cx<float> w[U];
@@ -628,7 +644,8 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
             }
             // x[n - RM] is no longer needed: its slot takes x[n + RM + 1]
             w[i] = B::ld_cx(rwork, row(n + RM + 1));
-            B::template st_cx<kStream>(cx<float>{ka * acc.x, ka * acc.y}, rwork, row(n));
+            B::template st_cx<kStream>(cx<float>{fmaf(ka, acc.x, kl * acc.x), fmaf(ka, acc.y, kl * acc.y)}, rwork,
+                                       row(n));
         }
     }
 }
