@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--grids", default=None,
                     help="ensemble: indices into the grid set to draw runs from (default: all five)")
+    ap.add_argument("--forms", default="graph,eager,plain,reference",
+                    help="ensemble: tick forms to time (graph, eager, plain, reference); one form alone is "
+                         "how each is profiled")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
@@ -527,31 +530,52 @@ def bench_ensemble(args):
         return (time.perf_counter() - a) / steps
 
     K = args.steps
-    t_graph = timed(lambda: eng.step(runs, graph=True, stream=stream), K)
-    t_batch = timed(lambda: eng.step(runs, graph=False, stream=stream), K)
-    t_plain = timed(lambda: eng.step_plain(runs, stream=stream), max(1, K // 2))
-    t_ref = reference_flow_tick(picks, nf, args.seed)
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if not forms or any(f not in ("graph", "eager", "plain", "reference") for f in forms):
+        raise SystemExit("--forms: choose from graph,eager,plain,reference")
+    t = {}
+    if "graph" in forms:
+        t["graph"] = timed(lambda: eng.step(runs, graph=True, stream=stream), K)
+    if "eager" in forms:
+        t["eager"] = timed(lambda: eng.step(runs, graph=False, stream=stream), K)
+    if "plain" in forms:
+        t["plain"] = timed(lambda: eng.step_plain(runs, stream=stream), max(1, K // 2))
+    if "reference" in forms:
+        t["reference"] = reference_flow_tick(picks, nf, args.seed)
     counts = {f"{s[2]}x{s[1]}x{s[0]}": int((picks == i).sum()) for i, s in enumerate(ENSEMBLE_GRIDS)}
     bytes_tick = 0.0
     for i, s in enumerate(ENSEMBLE_GRIDS):
         import vfd
-        p = vfd.Plan(s)
+        p = vfd.Plan(s, kspec_format=vfd.KSPEC_SEPARABLE)
         bytes_tick += p.info.bytes_per_field_tick * nf * counts[f"{s[2]}x{s[1]}x{s[0]}"]
-    out = _header(args, "voxel-updates/s, batched ensemble tick (64 uneven runs x 8 fields)", vox / t_graph,
-                  "voxel-updates/s", t_graph * 1e3,
+    head = forms[0]
+    out = _header(args, f"voxel-updates/s, ensemble tick (64 uneven runs x 8 fields), {head} form", vox / t[head],
+                  "voxel-updates/s", t[head] * 1e3,
                   {"workload": f"{args.runs} resident runs x {nf} FP32 fields, grids drawn from "
                                "{64x64x32,128x128x64,256x128x64,256x256x64,512x256x64}, one tick per step",
-                   "runs_per_grid": counts, "voxels_per_tick": vox})
-    out["batched_graph"] = {"ms_per_tick": t_graph * 1e3, "voxel_updates_s": vox / t_graph}
-    out["batched_eager"] = {"ms_per_tick": t_batch * 1e3, "voxel_updates_s": vox / t_batch}
-    out["plain_per_field"] = {"ms_per_tick": t_plain * 1e3, "voxel_updates_s": vox / t_plain}
-    out["speedup_vs_plain"] = t_plain / t_graph
-    out["reference_flow"] = {"ms_per_tick": t_ref * 1e3, "voxel_updates_s": vox / t_ref,
-                             "what": "the reference's per-field drop-in flow on these kernels: pinned host field -> "
-                                     "device, 5 pass launches each followed by a device synchronization, device -> "
-                                     "host, one field at a time (Diffuser.cpp:300-421, Diffusion.cu:1802-1870)"}
-    out["speedup_vs_reference_flow"] = t_ref / t_graph
-    out["algorithmic_gbs"] = bytes_tick / t_graph / 1e9
+                   "runs_per_grid": counts, "voxels_per_tick": vox, "forms": forms})
+    names = {"graph": "batched_graph", "eager": "batched_eager", "plain": "plain_per_field",
+             "reference": "reference_flow"}
+    for f, tt in t.items():
+        out[names[f]] = {"ms_per_tick": tt * 1e3, "voxel_updates_s": vox / tt, "algorithmic_gbs": bytes_tick / tt / 1e9}
+    if "reference" in t:
+        out["reference_flow"]["what"] = ("the reference's per-field drop-in flow on these kernels: pinned host field "
+                                         "-> device, 5 pass launches each followed by a device synchronization, "
+                                         "device -> host, one field at a time (Diffuser.cpp:300-421, "
+                                         "Diffusion.cu:1802-1870)")
+    if "graph" in t and "plain" in t:
+        out["speedup_vs_plain"] = t["plain"] / t["graph"]
+    if "graph" in t and "reference" in t:
+        out["speedup_vs_reference_flow"] = t["reference"] / t["graph"]
+    # every pass streams its arrays once: the bytes of a tick and the time they
+    # take at the measured streaming bandwidth (16-byte copy kernel)
+    out["algorithmic_bytes_per_tick"] = bytes_tick
+    out["algorithmic_gbs"] = bytes_tick / t[head] / 1e9
+    copy = copy_bandwidth(vfd, torch, stream)
+    out["copy_gbs"] = copy
+    out["hbm_floor_ms"] = bytes_tick / (copy * 1e9) * 1e3
+    if "plain" in t:
+        out["max_speedup_vs_plain_at_floor"] = t["plain"] / (bytes_tick / (copy * 1e9))
     out["setup_s"] = setup_s
     out["engine"] = eng.stats()
     eng.close()
