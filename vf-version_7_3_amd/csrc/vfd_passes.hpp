@@ -47,6 +47,9 @@ template <typename T, int Q, int V = 0> struct RowCfg {
 template <typename T, int Q, int V = 0, int RS = 2> struct ColCfg {
     static constexpr int MAXR = var_maxr(V);
     static constexpr int TPF = tpf_for(Q, MAXR);
+    // (a 144 KB cap for P >= 1024, 8-column tiles of 1536- and 1152-long
+    // columns at one work-group per CU, was slower than 4-column tiles at two:
+    // 0.59 - 0.66 ms against 0.38 - 0.40 ms per production y pass)
     static constexpr int LDS_CAP = 72 * 1024;
     // columns per block: the variant's thread cap, but never narrower than one
     // 128-byte line of a row (narrower tiles fetch whole lines and use part)
