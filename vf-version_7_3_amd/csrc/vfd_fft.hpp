@@ -406,6 +406,32 @@ template <bool WAVE> __device__ __forceinline__ void stage_sync() {
     }
 }
 
+// Mirror pairs by wavefront exchange.  In a wave-local transform whose slots
+// are all busy, element k = t + TPF*b + TPF*NB*r sits at lane t of its row's
+// lane group, slot (b, r) (t < TPF, b < NB, r < R).  Its mirror L - k (mod L)
+// sits at lane (TPF - t) % TPF, slot (NB-1-b, R-1-r), for t > 0; lane 0 holds
+// its own mirrors, at slot (0, R-r) for b = 0 < r and (NB-b, R-1-r) for b > 0
+// (slot (0, 0), element 0, is its own mirror).  A lane reads the partner's
+// register with ds_bpermute: one instruction per 32-bit word, no LDS storage
+// and no barrier.
+template <int NB, int R> struct MirrorSlots {
+    static constexpr int other(int b, int r) { return (NB - 1 - b) * R + (R - 1 - r); }  // lanes t > 0
+    static constexpr int lane0(int b, int r) {                                           // lane t = 0
+        return b == 0 ? (r == 0 ? 0 : R - r) : (NB - b) * R + (R - 1 - r);
+    }
+};
+// byte address for ds_bpermute of the mirror lane of lane t in its group
+template <int TPF> __device__ __forceinline__ int mirror_lane_addr(int t) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    return (lane - t + (TPF - t) % TPF) << 2;
+}
+__device__ __forceinline__ float lane_read(float v, int addr) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float2 lane_read(float2 v, int addr) {
+    return float2{lane_read(v.x, addr), lane_read(v.y, addr)};
+}
+
 template <typename T, int Q, int TPF, bool INV, bool REV, int S, int NS, int MAXR, bool WAVE = false>
 struct StageRunner {
     static constexpr RadixPlan PL = make_plan(Q, REV, MAXR);

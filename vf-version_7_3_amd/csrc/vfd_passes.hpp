@@ -39,6 +39,13 @@ template <typename T, int Q, int V = 0> struct RowCfg {
     // each row's transform and LDS row belong to one wavefront: its stages
     // synchronize that wave alone (stage_sync)
     static constexpr bool WS = wave_local<TPF>();
+    // x inverse: the real-FFT mirror pairs (A_k, A_{L-k}) are exchanged across
+    // lanes (ds_bpermute) instead of loaded twice: FP32, wave-local rows, every
+    // first-stage slot busy (MirrorSlots).  (The x forward pass's post-process
+    // exchange done the same way held all 16 outputs across it: 162 -> 170
+    // VGPRs, 3 -> 2 waves per SIMD, 1.33 -> 1.37 ms; it keeps the LDS exchange.)
+    using Sh_ = FftShape<T, Q, TPF, false, MAXR>;
+    static constexpr bool MIRROR = sizeof(T) == 4 && WS && (Q / Sh_::R0) % TPF == 0;
     // twiddle table W_Px (2Q entries) staged in LDS when it fits beside the rows
     static constexpr bool TW_LDS = (2 * Q + G * PITCH) * (int)sizeof(cx<T>) <= 64 * 1024;
     static constexpr int TW_N = TW_LDS ? 2 * Q : 1;
@@ -146,6 +153,13 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 v[b * Sh::R0 + r] = val;
             }
         fft_regs<T, L, C::TPF, false, false, C::MAXR, C::WS>(t, v, lds, tw, 2);
+        const O dst = (O)((long)z * g.plane + (long)y * g.ld) * CB;
+        auto post = [&](int k, cx<T> zk, cx<T> zc) {
+            const cx<T> e = cscale(cadd(zk, zc), T(0.5));
+            const cx<T> d = cscale(csub(zk, zc), T(0.5));
+            const cx<T> wd = cmul(d, tw[k]);  // W_Px^k d
+            return cx<T>{e.x + wd.y, e.y - wd.x};  // e - i W d
+        };
         stage_sync<C::WS>();
 #pragma unroll
         for (int b = 0; b < Sh::NBL; ++b)
@@ -155,14 +169,10 @@ xfwd_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
                 if (k >= 0) lds.at(k) = v[b * Sh::RL + r];
             }
         stage_sync<C::WS>();
-        const O dst = (O)((long)z * g.plane + (long)y * g.ld) * CB;
         for (int k = t; k <= L; k += C::TPF) {
             const cx<T> zk = lds.at(k == L ? 0 : k);
             const cx<T> zc = cconj(lds.at(k == 0 ? 0 : L - k));
-            const cx<T> e = cscale(cadd(zk, zc), T(0.5));
-            const cx<T> d = cscale(csub(zk, zc), T(0.5));
-            const cx<T> wd = cmul(d, tw[k]);  // W_Px^k d
-            B::template st_cx<kStream>(cx<T>{e.x + wd.y, e.y - wd.x}, rwork, live ? dst + (O)k * CB : B::OOB);  // e - i W d
+            B::template st_cx<kStream>(post(k, zk, zc), rwork, live ? dst + (O)k * CB : B::OOB);
         }
         // the next item's first LDS store happens after a barrier inside fft_regs
     }
@@ -214,18 +224,43 @@ xinv_kernel(const FieldBatch fb, GridDesc g, const cx<T>* __restrict__ twg, int 
         const O srow = (O)((long)z * g.plane + (long)y * g.ld) * CB;
 
         cx<T> v[Sh::E];
+        auto pre = [&](int k, cx<T> ak, cx<T> ac) {
+            const cx<T> xe = cadd(ak, ac);
+            const cx<T> xo = cmulc(csub(ak, ac), tw[k]);  // (A - A*) W^-k
+            return cx<T>{xe.x - xo.y, xe.y + xo.x};       // Xe + i Xo
+        };
+        if constexpr (C::MIRROR) {
+            // each A_k is loaded once; A_{L-k} comes from the mirror lane's
+            // registers, A_L (the mirror of k = 0) by one more load on lane 0
+            using M = MirrorSlots<Sh::NB0, Sh::R0>;
+            const int maddr = mirror_lane_addr<C::TPF>(t);
+            cx<T> a[Sh::NB0 * Sh::R0];
 #pragma unroll
-        for (int b = 0; b < Sh::NB0; ++b)
+            for (int b = 0; b < Sh::NB0; ++b)
 #pragma unroll
-            for (int r = 0; r < Sh::R0; ++r) {
-                const int k = Sh::in_elem(t, b, r);
-                const bool ok = k >= 0 && live;
-                const cx<T> ak = B::ld_cx(rwork, ok ? srow + (O)k * CB : B::OOB);
-                const cx<T> ac = cconj(B::ld_cx(rwork, ok ? srow + (O)(L - k) * CB : B::OOB));
-                const cx<T> xe = cadd(ak, ac);
-                const cx<T> xo = cmulc(csub(ak, ac), tw[ok ? k : 0]);  // (A - A*) W^-k
-                v[b * Sh::R0 + r] = cx<T>{xe.x - xo.y, xe.y + xo.x};    // Xe + i Xo
-            }
+                for (int r = 0; r < Sh::R0; ++r)
+                    a[b * Sh::R0 + r] = B::ld_cx(rwork, live ? srow + (O)Sh::in_elem(t, b, r) * CB : B::OOB);
+            const cx<T> aL = B::ld_cx(rwork, live && t == 0 ? srow + (O)L * CB : B::OOB);
+#pragma unroll
+            for (int b = 0; b < Sh::NB0; ++b)
+#pragma unroll
+                for (int r = 0; r < Sh::R0; ++r) {
+                    cx<T> m = lane_read(a[M::other(b, r)], maddr);
+                    if (t == 0) m = (b == 0 && r == 0) ? aL : a[M::lane0(b, r)];
+                    v[b * Sh::R0 + r] = pre(Sh::in_elem(t, b, r), a[b * Sh::R0 + r], cconj(m));
+                }
+        } else {
+#pragma unroll
+            for (int b = 0; b < Sh::NB0; ++b)
+#pragma unroll
+                for (int r = 0; r < Sh::R0; ++r) {
+                    const int k = Sh::in_elem(t, b, r);
+                    const bool ok = k >= 0 && live;
+                    const cx<T> ak = B::ld_cx(rwork, ok ? srow + (O)k * CB : B::OOB);
+                    const cx<T> ac = cconj(B::ld_cx(rwork, ok ? srow + (O)(L - k) * CB : B::OOB));
+                    v[b * Sh::R0 + r] = pre(ok ? k : 0, ak, ac);
+                }
+        }
         fft_regs<T, L, C::TPF, true, false, C::MAXR, C::WS>(t, v, lds, tw, 2);
         T addc = T(0);
         if (g.pad_mode == PAD_CONSTANT_VF) {
