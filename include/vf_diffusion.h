@@ -215,6 +215,18 @@ void modulate3D(fComplex* d_Dst, fComplex* d_Src, int fftD, int fftH, int fftW, 
 void complexPower(fComplex* d_Dst, int fftD, int fftH, int fftW, int padding, int n);
                                                                                   /* .cu:512-530 */
 
+/* ---- the 2D operator (convolutionFFT_common.h:356-380, the !MODEL_3D build) ---- */
+/* Diffusion.cu:303-675: the 5-point stencil's kernel after kernelRadius
+ * convolution rounds on the FFT box; the centred windowH x windowW window
+ * into d_Window. */
+bool computeKernel(float* d_Window, int kernelRadius, float lambda, float gamma, float dt, c_ctx cctx);
+/* Diffusion.cu:677-771: the packed spectrum (the C2C transform of the padded
+ * kernel read as FFTH x FFTW/2 complex) of the chem_cctx KH x KW window. */
+bool computeKernelSpectrum(fComplex* d_KernelSpectrum, float* d_Kernel, c_ctx kernel_cctx, c_ctx chem_cctx);
+/* Diffusion.cu:773-907: constant-VF pad, convolution with the packed
+ * spectrum's kernel, crop; in place on d_Data (DH x DW). */
+bool fftDiffuse2D(float* d_Data, fComplex* d_KernelSpectrum0, c_ctx cctx, int epiBoundary, float baseChem);
+
 /* ---- device helpers: convolutionFFT.cu (2D, the !MODEL_3D build) ---- */
 void padData2D(pad_t pt, float* d_PaddedData, float* d_Data, int fftH, int fftW, int dataH, int dataW,
                int kernelH, int kernelW, int kernelY, int kernelX, int epiBoundary, float baseChem);

@@ -475,3 +475,96 @@ def test_release_workspaces_then_diffuse_again(vfd, orc, leg):
         leg.lib.vfdReleaseWorkspaces.restype = ctypes.c_int
         assert leg.lib.vfdReleaseWorkspaces() == 0, vfd.last_error()
     assert torch.equal(outs[0], outs[1])
+
+
+# ---------------------------------------------------------------- 2D operator
+def _cc2d(leg, H, W, fH, fW, KH, KW, KY, KX, win=(1, 1)):
+    cc = leg.CCtx()
+    cc.DD, cc.DH, cc.DW = 1, H, W
+    cc.FFTD, cc.FFTH, cc.FFTW = 1, fH, fW
+    cc.KD, cc.KH, cc.KW = 1, KH, KW
+    cc.KZ, cc.KY, cc.KX = 0, KY, KX
+    cc.windowD, cc.windowH, cc.windowW = 1, win[0], win[1]
+    return cc
+
+
+@pytest.mark.parametrize("H,W,fH,fW,KY,KX,R,win", [
+    (16, 20, 32, 32, 1, 1, 6, (9, 9)),
+    (15, 18, 24, 36, 1, 1, 4, (7, 5)),
+    (12, 12, 16, 16, 0, 2, 3, (5, 5)),      # stencil centre off its middle: a phase in S^
+    (10, 14, 16, 32, 1, 1, 0, (3, 3)),      # no convolution round: the stencil itself
+])
+def test_computeKernel2D_brute_force(vfd, orc, leg, H, W, fH, fW, KY, KX, R, win):
+    """computeKernel (Diffusion.cu:303-675) against a brute-force FP64 restatement:
+    the stencil at the domain centre, clamp-to-border pad once (oracle), R
+    circular convolutions with the stencil by shifted sums on the FFT box,
+    crop, centred window (oracle)."""
+    import torch
+    print(f"case computeKernel2D H,W={H},{W} box={fH},{fW} centre={KY},{KX} R={R}", flush=True)
+    lam, gamma, dt = np.float32(0.11), np.float32(0.002), np.float32(1.5)
+    cc = _cc2d(leg, H, W, fH, fW, 3, 3, KY, KX, win)
+    d_win = torch.full(win, -1.0, dtype=torch.float32, device="cuda")
+    assert leg.lib.computeKernel(ctypes.c_void_p(d_win.data_ptr()), R, lam, gamma, dt, cc), vfd.last_error()
+    c0 = 1.0 - 4.0 * float(lam) - float(gamma) * float(dt)
+    hk = np.zeros((3, 3))
+    hk[0, 1] = hk[1, 0] = hk[1, 2] = hk[2, 1] = float(lam)
+    hk[1, 1] = c0
+    d0 = np.zeros((H, W))
+    d0[H // 2 - 1:H // 2 + 2, W // 2 - 1:W // 2 + 2] = hk
+    p = orc.pad_clamp(d0[None], (1, fH, fW), 0, KY, KX)[0]
+    for _ in range(R):
+        p = sum(hk[i, j] * np.roll(p, (i - KY, j - KX), axis=(0, 1)) for i in range(3) for j in range(3))
+    ref = orc.extract_center(p[:H, :W][None], (1,) + win)[0]
+    assert _rel(d_win.cpu().numpy(), ref) <= 1e-6
+
+
+def test_computeKernelSpectrum2D_packed(vfd, orc, leg):
+    """computeKernelSpectrum (Diffusion.cu:677-771): the C2C transform of the
+    padded kernel read as FFTH x FFTW/2 complex, against numpy's fft2."""
+    import torch
+    fH, fW, KH, KW, KY, KX = 24, 32, 5, 7, 2, 3
+    k = np.random.default_rng(5).random((KH, KW)).astype(np.float32)
+    cc = _cc2d(leg, 8, 8, fH, fW, KH, KW, KY, KX)
+    out = torch.zeros((fH, fW // 2), dtype=torch.complex64, device="cuda")
+    kd = _t(k)
+    assert leg.lib.computeKernelSpectrum(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(kd.data_ptr()), cc, cc), \
+        vfd.last_error()
+    p = orc.pad_kernel(k[None], (1, fH, fW), 0, KY, KX)[0]
+    ref = np.fft.fft2(p[:, 0::2] + 1j * p[:, 1::2])
+    got = out.cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"  packed spectrum rel-L2 {err:.3e}", flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("H,W,fH,fW,epi,base", [(20, 26, 32, 48, 22, 0.3), (17, 9, 24, 16, 100, 1.25), (9, 12, 16, 16, 0, 0.5)])
+def test_fftDiffuse2D(vfd, orc, leg, H, W, fH, fW, epi, base):
+    """fftDiffuse2D (Diffusion.cu:773-907) with a packed spectrum of a
+    non-symmetric 5x5 kernel: the constant-VF pad (the bit-exact 2D pad helper),
+    numpy's circular convolution on the box, crop; and the same through the
+    spectrum this library's computeKernelSpectrum builds."""
+    import torch
+    print(f"case fftDiffuse2D H,W={H},{W} box={fH},{fW} epi={epi}", flush=True)
+    rng = np.random.default_rng(H * W)
+    u = rng.random((H, W)).astype(np.float32)
+    k = rng.random((5, 5)).astype(np.float32)
+    kp = orc.pad_kernel(k[None], (1, fH, fW), 0, 2, 2)[0]
+    packed = np.fft.fft2(kp[:, 0::2] + 1j * kp[:, 1::2]).astype(np.complex64)
+    cc = _cc2d(leg, H, W, fH, fW, 5, 5, 2, 2)
+    ud = _t(u)
+    pad = torch.empty((fH, fW), dtype=torch.float32, device="cuda")
+    P = ctypes.c_void_p
+    leg.lib.padDataConstantVF(P(pad.data_ptr()), P(ud.data_ptr()), fH, fW, H, W, 5, 5, 2, 2, epi, base)
+    torch.cuda.synchronize()
+    p = pad.cpu().numpy().astype(np.float64)
+    ref = np.real(np.fft.ifft2(np.fft.fft2(p) * np.fft.fft2(kp)))[:H, :W]
+    for src in ("numpy", "library"):
+        d = ud.clone()
+        if src == "numpy":
+            ks = _t(packed)
+        else:
+            ks = torch.zeros((fH, fW // 2), dtype=torch.complex64, device="cuda")
+            assert leg.lib.computeKernelSpectrum(P(ks.data_ptr()), P(_t(k).data_ptr()), cc, cc), vfd.last_error()
+        assert leg.lib.fftDiffuse2D(P(d.data_ptr()), P(ks.data_ptr()), cc, epi, ctypes.c_float(base)), vfd.last_error()
+        print(f"  spectrum from {src}:", end="", flush=True)
+        assert _rel(d.cpu().numpy(), ref) <= 1e-5

@@ -155,6 +155,71 @@ __global__ void stencil_power_multiply(double2* spec, int fD, int fH, int fW, in
     spec[i] = double2{scale * (v.x * pr - v.y * pi), scale * (v.x * pi + v.y * pr)};
 }
 
+// 2D kernel build: x S^n * scale for the 5-point stencil centred at (KY, KX)
+// of the padded kernel, on every kz plane of a z-constant spectrum:
+// S(ty, tx) = exp(-i (ty (1 - KY) + tx (1 - KX))) (c0 + 2 lambda (cos ty + cos tx))
+__global__ void stencil2d_power_multiply(double2* spec, int fD, int fH, int fW, int hx, double lambda, double c0,
+                                         int n, int KY, int KX, double scale) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)fD * fH * hx;
+    if (i >= total) return;
+    const int kx = (int)(i % hx);
+    const int ky = (int)((i / hx) % fH);
+    const double tp = 6.283185307179586476925286766559005768;
+    const double ty = tp * ky / fH, tx = tp * kx / fW;
+    const double mag = pow(c0 + 2.0 * lambda * (cos(ty) + cos(tx)), (double)n);  // signed for odd n
+    double sp, cp;
+    sincos(-(double)n * (ty * (1 - KY) + tx * (1 - KX)), &sp, &cp);
+    const double pr = mag * cp, pi = mag * sp;
+    const double2 v = spec[i];
+    spec[i] = double2{scale * (v.x * pr - v.y * pi), scale * (v.x * pi + v.y * pr)};
+}
+
+// The 2D path's kernel spectrum is the C2C transform Z of the padded kernel p
+// (H x W real) read as H x W/2 complex z[h][m] = p[h][2m] + i p[h][2m+1]
+// (Diffusion.cu:677-771, spProcess2D at convolutionFFT.cuh:948-1021).  With
+// M = W/2, E and O the transforms of the even and odd columns and P the R2C
+// spectrum of p:  Z = E + i O,  E(a,b) = (P(a,b) + P(a,b+M)) / 2,
+// O(a,b) = W_W^-b (P(a,b) - P(a,b+M)) / 2,  P(a,b+M) = conj P(-a, M-b) (b > 0);
+// and back, P(a,k) = E(a,k mod M) + W_W^k O(a,k mod M) with
+// E = (Z(a,b) + conj Z(-a,-b)) / 2,  O = -i (Z(a,b) - conj Z(-a,-b)) / 2.
+__global__ void standard_to_packed2d(float2* Z, const double2* P, int H, int W, int hx, double s) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = W / 2;
+    if (i >= (long)H * M) return;
+    const int a = (int)(i / M), b = (int)(i % M);
+    const double2 pa = P[(long)a * hx + b];
+    double2 pb = P[(long)a * hx + M];
+    if (b > 0) {
+        const double2 q = P[(long)((H - a) % H) * hx + (M - b)];
+        pb = double2{q.x, -q.y};
+    }
+    const double tp = 6.283185307179586476925286766559005768;
+    double sw, cw;
+    sincos(tp * b / W, &sw, &cw);  // W_W^-b = exp(+2 pi i b / W)
+    const double ex = 0.5 * (pa.x + pb.x), ey = 0.5 * (pa.y + pb.y);
+    const double dx = 0.5 * (pa.x - pb.x), dy = 0.5 * (pa.y - pb.y);
+    const double ox = dx * cw - dy * sw, oy = dx * sw + dy * cw;
+    Z[i] = float2{(float)(s * (ex - oy)), (float)(s * (ey + ox))};  // E + i O
+}
+
+__global__ void packed_to_standard2d(float2* P, const float2* Z, int fD, int H, int W) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = W / 2, hx = M + 1;
+    if (i >= (long)H * hx) return;
+    const int a = (int)(i / hx), k = (int)(i % hx);
+    const int b = k % M;
+    const float2 z1 = Z[(long)a * M + b];
+    const float2 z2 = Z[(long)((H - a) % H) * M + (M - b) % M];
+    const double ex = 0.5 * ((double)z1.x + z2.x), ey = 0.5 * ((double)z1.y - z2.y);  // (z1 + conj z2) / 2
+    const double ox = 0.5 * ((double)z1.y + z2.y), oy = -0.5 * ((double)z1.x - z2.x); // -i (z1 - conj z2) / 2
+    const double tp = 6.283185307179586476925286766559005768;
+    double sw, cw;
+    sincos(-tp * k / W, &sw, &cw);  // W_W^k
+    const float2 v{(float)(ex + ox * cw - oy * sw), (float)(ey + ox * sw + oy * cw)};
+    for (int z = 0; z < fD; ++z) P[(long)z * H * hx + i] = v;
+}
+
 __global__ void to_fcomplex(float2* dst, const double2* src, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -900,6 +965,158 @@ bool computeKernel3DBatch(int kernelRadius, float lambda[N_CHEM], float gamma[N_
         ok = computeKernel3D(kernelRadius, lambda[ic], gamma[ic], dt, kern_cctx, 0, 0, (short)ic) &&
              computeKernelSpectrum3D(kern_cctx, chem_cctx, (short)ic);
     }
+    return ok;
+}
+
+// ------------------------------------------------------------ 2D path
+// The 2D entry points (Diffusion.cu:303-907, the !MODEL_3D build) run on the
+// 3D passes: an H x W array on an (fftH, fftW) box is plane 0 of a box two
+// planes deep, and a 2D kernel k is k (x) delta(z), whose spectrum is the 2D
+// one on both kz planes.  The kernel build works on a z-constant array (both
+// planes equal), whose spectrum is twice the 2D one on plane kz = 0.
+
+// computeKernel (Diffusion.cu:303-675): the 5-point stencil (lambda at the four
+// neighbours, 1 - 4 lambda - gamma dt at the centre of its KH x KW array) is
+// placed at the centre of the H x W domain, padded once to the FFT box by
+// clamp-to-border, convolved kernelRadius times with the stencil on the box
+// (the reference repeats R2C, x S^ / (fftH fftW), C2R on the padded array),
+// cropped to the domain, and the centred window is written to d_Window.  Here
+// the kernelRadius convolutions are one multiplication by S^kernelRadius, in
+// FP64.  The reference copies 2 (KH/2) + 1 stencil rows and columns (one past
+// the array for even KH or KW); here only the KH x KW array is copied.
+bool computeKernel(float* d_Window, int kernelRadius, float lambda, float gamma, float dt, c_ctx cctx) {
+    const int H = cctx.DH, W = cctx.DW, fH = cctx.FFTH, fW = cctx.FFTW;
+    const int wH = cctx.windowH, wW = cctx.windowW, KH = cctx.KH, KW = cctx.KW;
+    if (!d_Window) return fail(set_error(VFD_ERR_INVALID, "computeKernel: null window"));
+    if (H < 3 || W < 3 || KH < 3 || KW < 3 || fH < H || fW < W || wH < 1 || wW < 1 || wH > H || wW > W ||
+        kernelRadius < 0)
+        return fail(set_error(VFD_ERR_INVALID, "computeKernel: bad kernel context dimensions"));
+    std::lock_guard<std::mutex> lk(g_mu);
+    vfd_plan_desc d{2, fH, fW, 2, fH, fW, VFD_F64, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    Plan plan;
+    if (int rc = plan_init(&d, &plan)) return fail(rc);
+    const long dn = (long)H * W, pn = (long)fH * fW, sn = 2L * fH * plan.hx;
+    const double lam = lambda, c0 = 1.0 - 4.0 * (double)lambda - (double)gamma * (double)dt;
+    std::vector<double> hk((size_t)KH * KW, 0.0);
+    hk[0 * KW + 1] = lam;
+    hk[1 * KW + 0] = lam;
+    hk[1 * KW + 2] = lam;
+    hk[2 * KW + 1] = lam;
+    hk[1 * KW + 1] = c0;
+    std::vector<double> h_data(dn, 0.0);
+    const int i0 = H / 2 - KH / 2, j0 = W / 2 - KW / 2;
+    for (int ki = 0; ki < KH && i0 + ki < H; ++ki)
+        for (int kj = 0; kj < KW && j0 + kj < W; ++kj)
+            if (i0 + ki >= 0 && j0 + kj >= 0) h_data[(long)(i0 + ki) * W + (j0 + kj)] = hk[ki * KW + kj];
+    double *d_data = nullptr, *d_pad = nullptr;
+    double2* d_spec = nullptr;
+    bool ok = hip_ok(hipMalloc(&d_data, dn * sizeof(double)), "kernel build alloc") &&
+              hip_ok(hipMalloc(&d_pad, 2 * pn * sizeof(double)), "kernel build alloc") &&
+              hip_ok(hipMalloc(&d_spec, sn * sizeof(double2)), "kernel build alloc") &&
+              hip_ok(hipMemcpy(d_data, h_data.data(), dn * sizeof(double), hipMemcpyHostToDevice), "kernel upload");
+    if (ok) {
+        hipLaunchKernelGGL(pad_clamp3d<double>, dim3(nblocks(pn)), dim3(kThreads), 0, nullptr, d_pad, d_data, 1, fH,
+                           fW, 1, H, W, 0, cctx.KY, cctx.KX);
+        ok = hip_ok(hipGetLastError(), "kernel build pad") &&
+             hip_ok(hipMemcpy(d_pad + pn, d_pad, pn * sizeof(double), hipMemcpyDeviceToDevice), "kernel build copy") &&
+             (plan_r2c(plan, d_pad, d_spec, nullptr) == VFD_OK || fail(-1));
+    }
+    if (ok) {
+        hipLaunchKernelGGL(stencil2d_power_multiply, dim3(nblocks(sn)), dim3(kThreads), 0, nullptr, d_spec, 2, fH, fW,
+                           plan.hx, lam, c0, kernelRadius, cctx.KY, cctx.KX, 1.0 / (2.0 * fH * fW));
+        ok = hip_ok(hipGetLastError(), "kernel build power") &&
+             (plan_c2r(plan, d_spec, d_pad, nullptr) == VFD_OK || fail(-1));
+    }
+    if (ok) {
+        hipLaunchKernelGGL(unpad3d<double>, dim3(nblocks(dn)), dim3(kThreads), 0, nullptr, d_data, d_pad, 1, H, W, fH,
+                           fW);
+        hipLaunchKernelGGL((extract_center3d<float, double>), dim3(nblocks((long)wH * wW)), dim3(kThreads), 0, nullptr,
+                           d_Window, d_data, 1, H, W, 1, wH, wW);
+        ok = hip_ok(hipGetLastError(), "extract window") && hip_ok(hipDeviceSynchronize(), "kernel build");
+    }
+    (void)hipFree(d_data);
+    (void)hipFree(d_pad);
+    (void)hipFree(d_spec);
+    return ok;
+}
+
+// computeKernelSpectrum (Diffusion.cu:677-771): the KH x KW window d_Kernel
+// (chem_cctx KH, KW, centre KY, KX) padded into the chem FFT box and stored as
+// the packed spectrum fftH x fftW/2 fComplex that fftDiffuse2D reads.
+bool computeKernelSpectrum(fComplex* d_KernelSpectrum, float* d_Kernel, c_ctx kernel_cctx, c_ctx chem_cctx) {
+    (void)kernel_cctx;
+    const int fH = chem_cctx.FFTH, fW = chem_cctx.FFTW, KH = chem_cctx.KH, KW = chem_cctx.KW;
+    if (!d_KernelSpectrum || !d_Kernel) return fail(set_error(VFD_ERR_INVALID, "computeKernelSpectrum: null buffer"));
+    if (KH < 1 || KW < 1 || KH > fH || KW > fW || fW % 2)
+        return fail(set_error(VFD_ERR_INVALID, "computeKernelSpectrum: bad kernel or FFT box"));
+    std::lock_guard<std::mutex> lk(g_mu);
+    vfd_plan_desc d{2, fH, fW, 2, fH, fW, VFD_F64, VFD_PAD_ZERO, VFD_KSPEC_REAL};
+    Plan plan;
+    if (int rc = plan_init(&d, &plan)) return fail(rc);
+    const long pn = (long)fH * fW, sn = 2L * fH * plan.hx;
+    double* d_pad = nullptr;
+    double2* d_spec = nullptr;
+    bool ok = hip_ok(hipMalloc(&d_pad, 2 * pn * sizeof(double)), "spectrum alloc") &&
+              hip_ok(hipMalloc(&d_spec, sn * sizeof(double2)), "spectrum alloc");
+    if (ok) {
+        hipLaunchKernelGGL(fill<double>, dim3(nblocks(pn)), dim3(kThreads), 0, nullptr, d_pad, pn, 0.0);
+        hipLaunchKernelGGL((pad_kernel3d<double, float>), dim3(nblocks((long)KH * KW)), dim3(kThreads), 0, nullptr,
+                           d_pad, d_Kernel, 1, fH, fW, 1, KH, KW, 0, chem_cctx.KY, chem_cctx.KX);
+        ok = hip_ok(hipGetLastError(), "pad kernel") &&
+             hip_ok(hipMemcpy(d_pad + pn, d_pad, pn * sizeof(double), hipMemcpyDeviceToDevice), "spectrum copy") &&
+             (plan_r2c(plan, d_pad, d_spec, nullptr) == VFD_OK || fail(-1));
+    }
+    if (ok) {
+        // plane kz = 0 of the z-constant spectrum is twice the 2D one
+        hipLaunchKernelGGL(standard_to_packed2d, dim3(nblocks((long)fH * (fW / 2))), dim3(kThreads), 0, nullptr,
+                           (float2*)d_KernelSpectrum, d_spec, fH, fW, plan.hx, 0.5);
+        ok = hip_ok(hipGetLastError(), "spectrum convert") && hip_ok(hipDeviceSynchronize(), "spectrum");
+    }
+    (void)hipFree(d_pad);
+    (void)hipFree(d_spec);
+    return ok;
+}
+
+// fftDiffuse2D (Diffusion.cu:773-907): constant-VF pad of the H x W field
+// (interior 0 < y < H-1, 0 < x < min(epiBoundary, W), baseChem elsewhere),
+// circular convolution on the FFT box with the kernel of the packed spectrum,
+// normalized, cropped back to H x W in place.  Buffers are allocated per call,
+// as the reference does.
+bool fftDiffuse2D(float* d_Data, fComplex* d_KernelSpectrum0, c_ctx cctx, int epiBoundary, float baseChem) {
+    const int H = cctx.DH, W = cctx.DW, fH = cctx.FFTH, fW = cctx.FFTW;
+    if (!d_Data || !d_KernelSpectrum0) return fail(set_error(VFD_ERR_INVALID, "fftDiffuse2D: null buffer"));
+    if (H < 1 || W < 1 || fH < H || fW < W || fW % 2)
+        return fail(set_error(VFD_ERR_INVALID, "fftDiffuse2D: bad data or FFT box"));
+    std::lock_guard<std::mutex> lk(g_mu);
+    vfd_plan_desc d{1, fH, fW, 2, fH, fW, VFD_F32, VFD_PAD_ZERO, VFD_KSPEC_COMPLEX_STD};
+    Plan plan;
+    if (int rc = plan_init(&d, &plan)) return fail(rc);
+    const long pn = (long)fH * fW, sn = 2L * fH * (fW / 2 + 1);
+    float* d_pad = nullptr;
+    float2* d_spec = nullptr;
+    void* d_work = nullptr;
+    bool ok = hip_ok(hipMalloc(&d_pad, pn * sizeof(float)), "fftDiffuse2D alloc") &&
+              hip_ok(hipMalloc(&d_spec, sn * sizeof(float2)), "fftDiffuse2D alloc") &&
+              hip_ok(hipMalloc(&d_work, plan.work_bytes), "fftDiffuse2D alloc");
+    if (ok) {
+        hipLaunchKernelGGL(pad_constant_vf2d, dim3(nblocks(pn)), dim3(kThreads), 0, nullptr, d_pad, d_Data, fH, fW, H,
+                           W, epiBoundary, baseChem);
+        hipLaunchKernelGGL(packed_to_standard2d, dim3(nblocks((long)fH * (fW / 2 + 1))), dim3(kThreads), 0, nullptr,
+                           d_spec, (const float2*)d_KernelSpectrum0, 2, fH, fW);
+        ok = hip_ok(hipGetLastError(), "fftDiffuse2D pad");
+    }
+    if (ok) {
+        vfd_field f{d_pad, d_pad, d_work, d_spec, 0.0, 0, 0};
+        ok = plan_diffuse(plan, &f, 1, nullptr) == VFD_OK || fail(-1);
+    }
+    if (ok) {
+        hipLaunchKernelGGL(unpad3d<float>, dim3(nblocks((long)H * W)), dim3(kThreads), 0, nullptr, d_Data, d_pad, 1, H,
+                           W, fH, fW);
+        ok = hip_ok(hipGetLastError(), "fftDiffuse2D unpad") && hip_ok(hipDeviceSynchronize(), "fftDiffuse2D");
+    }
+    (void)hipFree(d_pad);
+    (void)hipFree(d_spec);
+    (void)hipFree(d_work);
     return ok;
 }
 

@@ -31,6 +31,13 @@ lib.computeKernelSpectrum3D.argtypes = [CCtx, CCtx, ctypes.c_short]
 lib.computeKernelSpectrum3D.restype = ctypes.c_bool
 lib.computeKernel3DBatch.argtypes = [_i, ctypes.POINTER(_f), ctypes.POINTER(_f), _f, CCtx, CCtx]
 lib.computeKernel3DBatch.restype = ctypes.c_bool
+# the 2D operator (the reference's !MODEL_3D build)
+lib.computeKernel.argtypes = [_p, _i, _f, _f, _f, CCtx]
+lib.computeKernel.restype = ctypes.c_bool
+lib.computeKernelSpectrum.argtypes = [_p, _p, CCtx, CCtx]
+lib.computeKernelSpectrum.restype = ctypes.c_bool
+lib.fftDiffuse2D.argtypes = [_p, _p, CCtx, _i, _f]
+lib.fftDiffuse2D.restype = ctypes.c_bool
 lib.vfdPlan3d.argtypes = [ctypes.POINTER(_i), _i, _i, _i, _i]
 lib.vfdDestroy.argtypes = [_i]
 lib.H2D.argtypes = [_i, ctypes.POINTER(CCtx), _i]
