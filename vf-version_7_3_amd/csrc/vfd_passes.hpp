@@ -387,8 +387,16 @@ __device__ __forceinline__ int zform_of_table(const void* kspec, unsigned long k
     return zform_of(BD::ld_r(tk, (typename BD::O)sep_g * 8u));
 }
 
+// Waves per SIMD the compiler must leave room for: the FP32 inverse pass of
+// 512-thread work-groups (both sub-sequences' inputs loaded up front) needs 131
+// VGPRs unconstrained, one work-group per CU; at 128 two fit
+template <typename T, int MODE, int NT> constexpr int col_waves_per_eu() {
+    return sizeof(T) == 4 && MODE == COL_INV && NT == 512 ? 4 : 1;
+}
+
 template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
 __global__ void __launch_bounds__((ColCfg<T, Q, V, RS>::NT))
+__attribute__((amdgpu_waves_per_eu((col_waves_per_eu<T, MODE, ColCfg<T, Q, V, RS>::NT>()))))
 col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     using C = ColCfg<T, Q, V, RS>;
     using ShF = FftShape<T, Q, C::TPF, false, C::MAXR>;                // forward / inverse-only plan
