@@ -46,6 +46,10 @@ Other workloads (one JSON line each, same fields where they apply):
                        drawn from that set plus 1024x512x128, admitted against a
                        200 GB HBM budget; continuous batching; throughput and
                        p50/p99 per-request latency.
+  --workload admit     serving with run churn: client threads open runs, tick
+                       them, close them; a budget (--budget-gb) smaller than the
+                       clients' runs makes opening runs wait for memory
+                       (vfd_run_create_wait); admission waits beside latency.
 
 Prints ONE JSON line.
 """
@@ -82,7 +86,7 @@ def parse():
     ap.add_argument("--kspec", default="separable", choices=["separable", "real"],
                     help="kernel-spectrum format: FP64 1D tables formed in the z pass, or the 3D real array")
     ap.add_argument("--workload", default="step",
-                    choices=["step", "ensemble", "serve", "copy", "precision", "update", "dropin"])
+                    choices=["step", "ensemble", "serve", "admit", "copy", "precision", "update", "dropin"])
     ap.add_argument("--ticks", type=int, default=100, help="precision: ticks of FP32 and FP64 side by side")
     ap.add_argument("--runs", type=int, default=64, help="ensemble/serve: resident runs")
     ap.add_argument("--grids", default=None,
@@ -92,8 +96,11 @@ def parse():
                          "how each is profiled")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: mean requests/s")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: seconds of arrivals")
-    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget (GB)")
+    ap.add_argument("--budget-gb", type=float, default=None,
+                    help="serve/admit: HBM budget in GB (default 200 for serve, 8 for admit)")
     ap.add_argument("--slots", type=int, default=4, help="serve: batches in flight")
+    ap.add_argument("--clients", type=int, default=16, help="admit: client threads, each opening runs in turn")
+    ap.add_argument("--ticks-per-run", type=int, default=10, help="admit: ticks each run is served before it closes")
     return ap.parse_args()
 
 
@@ -590,6 +597,7 @@ def bench_serve(args):
     import vfd.engine as ve
     rng = np.random.default_rng(args.seed)
     nf = args.fields
+    args.budget_gb = args.budget_gb or 200.0
     budget = int(args.budget_gb * 1e9)
     eng = ve.Engine(budget)
     runs, vox_of, refused = [], {}, 0
@@ -650,6 +658,81 @@ def bench_serve(args):
                          "mean": float(np.mean(lat)), "max": float(np.max(lat))}
     out["engine"] = st
     eng.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_admit(args):
+    """Serving with run churn: client threads each open a run (a grid drawn from
+    the serving set), tick it --ticks-per-run times through the server, close
+    it and open the next; the HBM budget holds only part of the clients' runs,
+    so an opening run waits in vfd_run_create_wait until closed runs free
+    memory.  Reports served throughput, per-tick p50/p99 latency and the
+    admission waits."""
+    import threading
+    import numpy as np
+    import vfd
+    import vfd.engine as ve
+    nf = args.fields
+    args.budget_gb = args.budget_gb or 8.0
+    budget = int(args.budget_gb * 1e9)
+    eng = ve.Engine(budget)
+    eng.serve_start(slots=args.slots, max_batch_fields=256)
+    # every grid once, so that first-use work (plans, autotuning) is not timed
+    for i, shape in enumerate(SERVE_GRIDS):
+        rid = eng.create_run_wait(shape, [2.0] * nf, [0.99] * nf, 600.0)
+        eng.wait(eng.submit(rid))
+        eng.destroy_run(rid)
+    lock = threading.Lock()
+    lat, waits, vox = [], [], [0.0]
+    errors = []
+    per_client = max(1, args.runs // args.clients)
+
+    def client(c):
+        rng = np.random.default_rng(args.seed + 7919 * c)
+        try:
+            for r in range(per_client):
+                shape = SERVE_GRIDS[int(rng.integers(0, len(SERVE_GRIDS)))]
+                s2, dc = _run_params(rng, nf)
+                t0 = time.perf_counter()
+                rid = eng.create_run_wait(shape, s2, dc, 600.0)
+                w = time.perf_counter() - t0
+                _fill_run(eng, rid, shape, nf, args.seed + 104729 * (c * per_client + r))
+                mine = []
+                for _ in range(args.ticks_per_run):
+                    mine.append(eng.wait(eng.submit(rid)))
+                eng.destroy_run(rid)
+                with lock:
+                    lat.extend(mine)
+                    waits.append(w * 1e3)
+                    vox[0] += float(np.prod(shape)) * nf * args.ticks_per_run
+        except Exception as e:  # reported after the join
+            errors.append(repr(e))
+
+    ths = [threading.Thread(target=client, args=(c,)) for c in range(args.clients)]
+    t0 = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.serve_stop()
+    eng.close()
+    if errors:
+        raise SystemExit(f"admit: client failed: {errors[0]}")
+    lat, waits = np.array(lat), np.array(waits)
+    out = _header(args, "voxel-updates/s served with run churn (admission waits for HBM); p50/p99 tick latency",
+                  vox[0] / wall, "voxel-updates/s", wall * 1e3 / max(1, len(lat)),
+                  {"workload": f"{args.clients} clients x {per_client} runs each, {args.ticks_per_run} ticks per run, "
+                               f"{nf} FP32 fields, grids from the ensemble set plus 1024x512x128, "
+                               f"HBM budget {args.budget_gb} GB", "slots": args.slots})
+    out["runs"] = int(len(waits))
+    out["ticks"] = int(len(lat))
+    out["latency_ms"] = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                         "max": float(lat.max())}
+    out["admission_wait_ms"] = {"p50": float(np.percentile(waits, 50)), "p99": float(np.percentile(waits, 99)),
+                                "max": float(waits.max())}
+    out["engine"] = st
     print(json.dumps(out), flush=True)
 
 
@@ -794,6 +877,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "admit":
+        return bench_admit(args)
     if args.workload == "copy":
         return bench_copy(args)
     if args.workload == "precision":

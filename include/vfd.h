@@ -245,6 +245,8 @@ typedef struct {
     size_t budget_bytes, used_bytes;
     int runs, shapes;
     uint64_t batches, ticks, requests_done, requests_pending;
+    size_t peak_used_bytes;            /* high-water mark of used_bytes */
+    uint64_t runs_admitted_after_wait; /* vfd_run_create_wait calls that waited for memory */
 } vfd_engine_stats;
 
 int vfd_engine_create(size_t hbm_budget_bytes, vfd_engine** engine);

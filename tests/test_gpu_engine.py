@@ -237,3 +237,31 @@ def test_admission_waits_for_memory(vfd, orc, eng_mod):
         assert not th.is_alive() and len(got) == 1
         st = eng.stats()
         assert st["runs"] == 2 and st["used_bytes"] <= st["budget_bytes"]
+        assert st["used_bytes"] <= st["peak_used_bytes"] <= st["budget_bytes"]
+        assert st["runs_admitted_after_wait"] == 1           # the timed-out waits admitted nothing
+
+
+def test_admission_frees_idle_work_buffers(vfd, orc, eng_mod):
+    """A batch grows its shape's pool of work buffers (one per field here);
+    once the batch is done those buffers are idle, and a run that fits only
+    without them is admitted after they are freed (each shape keeps one)."""
+    import torch
+    print("case admission_frees_idle_work_buffers", flush=True)
+    shape, nf = (32, 64, 64), 4
+    run = eng_mod.run_bytes(shape, nf)
+    work = vfd.Plan(shape, kspec_format=vfd.KSPEC_SEPARABLE).info.work_bytes
+    s2, dc = [2.0] * nf, [0.99] * nf
+    with eng_mod.Engine(2 * run + 2 * nf * work + run // 2) as eng:
+        a = eng.create_run(shape, s2, dc)
+        b = eng.create_run(shape, s2, dc)
+        eng.step([a, b])
+        torch.cuda.synchronize()
+        grown = eng.stats()["used_bytes"] - 2 * run
+        assert grown % work == 0 and grown // work == 2 * nf   # one work buffer per field of the batch
+        c = eng.create_run(shape, s2, dc)                      # fits only without the idle buffers
+        st = eng.stats()
+        assert st["used_bytes"] == 3 * run + work and st["runs"] == 3
+        assert st["runs_admitted_after_wait"] == 0
+        eng.step([a, b, c])                                     # still ticks, with the buffers it can get
+        torch.cuda.synchronize()
+        assert eng.stats()["used_bytes"] <= st["budget_bytes"]

@@ -138,7 +138,8 @@ struct vfd_engine {
     std::mutex mu;
     std::condition_variable cv;
     int dev = 0;
-    size_t budget = 0, used = 0;
+    size_t budget = 0, used = 0, peak = 0;
+    uint64_t admit_waits = 0;  // runs admitted after waiting for memory
     std::map<ShapeKey, Shape> shapes;
     std::map<int, Run> runs;
     int next_run = 1;
@@ -182,6 +183,7 @@ int acquire(vfd_engine* e, Shape& s, int want, Held* h) {
             break;
         }
         e->used += s.plan.work_bytes;
+        e->peak = std::max(e->peak, e->used);
         s.pool.push_back(p);
         s.held.push_back(1);
         h->idx.push_back((int)s.pool.size() - 1);
@@ -192,6 +194,26 @@ int acquire(vfd_engine* e, Shape& s, int want, Held* h) {
 void release(Held& h) {
     for (int i : h.idx) h.shape->held[i] = 0;
     h.idx.clear();
+}
+
+// Give the idle work buffers at the end of each shape's pool back to the
+// budget (each shape keeps its first one): pools that earlier batches grew
+// would otherwise hold memory an admission is waiting for.  Only trailing
+// entries go, so the indices that in-flight batches and graphs hold stay valid.
+bool trim_pools(vfd_engine* e) {
+    bool any = false;
+    for (auto& kv : e->shapes) {
+        Shape& s = kv.second;
+        while (s.pool.size() > 1 && !s.held.back()) {
+            if (!any) (void)hipDeviceSynchronize();  // released buffers may still be read by queued ticks
+            any = true;
+            (void)hipFree(s.pool.back());
+            s.pool.pop_back();
+            s.held.pop_back();
+            e->used -= s.plan.work_bytes;
+        }
+    }
+    return any;
 }
 
 // Group the listed runs' fields by shape and reserve their work buffers
@@ -353,6 +375,32 @@ void server_loop(vfd_engine* e) {
     }
 }
 
+// One run of a shape leaves (or a run being built gives up): after the last
+// one the shape's work buffers go to `frees` (their bytes added to `bytes`) and
+// the shape is dropped.  Buffers a batch or graph still holds keep the shape.
+void leave_shape(vfd_engine* e, const ShapeKey& key, std::vector<void*>* frees, size_t* bytes) {
+    auto sit = e->shapes.find(key);
+    if (sit == e->shapes.end() || --sit->second.nruns > 0) return;
+    for (char c : sit->second.held)
+        if (c) return;
+    for (void* p : sit->second.pool) {
+        frees->push_back(p);
+        *bytes += sit->second.plan.work_bytes;
+    }
+    e->shapes.erase(sit);
+}
+
+// Free device memory without the engine lock (hipFree waits for the device's
+// queued work; the server keeps collecting and launching batches meanwhile),
+// then return the bytes to the budget and wake admissions waiting for them.
+void free_and_return(vfd_engine* e, const std::vector<void*>& frees, size_t bytes) {
+    if (!frees.empty()) (void)hipDeviceSynchronize();
+    for (void* p : frees) (void)hipFree(p);
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->used -= bytes;
+    e->cv.notify_all();
+}
+
 }  // namespace
 
 // ====================================================================== C-ABI
@@ -399,33 +447,55 @@ size_t vfd_run_bytes(const vfd_run_desc* d) {
 // Admit a run: wait_s <= 0 refuses at once when the run does not fit the
 // budget; otherwise the call waits up to wait_s seconds for destroyed runs to
 // free the memory (a run larger than the whole budget is refused at once).
+// The run's bytes are reserved under the engine lock; its fields and kernel
+// spectra are then allocated and written without it.
 static int run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id, double wait_s) {
     if (!e || !d || !run_id || !d->sigma2 || !d->decay)
         return set_error(VFD_ERR_INVALID, "run_create: null argument");
     if (d->nfields < 1 || d->nfields > kMaxRunFields)
         return set_error(VFD_ERR_INVALID, "run_create: nfields %d outside 1..%d", d->nfields, kMaxRunFields);
-    std::unique_lock<std::mutex> lk(e->mu);
     const ShapeKey key{d->nz, d->ny, d->nx, d->precision};
-    const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
-                                             std::chrono::duration<double>(wait_s > 0 ? wait_s : 0.0));
-    std::map<ShapeKey, Shape>::iterator sit;
-    bool new_shape;
     Plan plan;
-    size_t need;
-    for (;;) {  // the shape may appear or go while this call waits
-        sit = e->shapes.find(key);
-        new_shape = sit == e->shapes.end();
-        if (new_shape) {
-            if (int rc = make_plan_for(key, &plan)) return rc;
-        } else {
-            plan = sit->second.plan;
+    size_t reserved = 0, reserved_work = 0;
+    {
+        std::unique_lock<std::mutex> lk(e->mu);
+        const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                                 std::chrono::duration<double>(wait_s > 0 ? wait_s : 0.0));
+        bool waited = false;
+        for (;;) {  // the shape may appear or go while this call waits
+            auto sit = e->shapes.find(key);
+            const bool new_shape = sit == e->shapes.end();
+            if (new_shape) {
+                if (int rc = make_plan_for(key, &plan)) return rc;
+            } else {
+                plan = sit->second.plan;
+            }
+            // a new shape also needs its first work buffer
+            const size_t need = run_bytes_for(plan, d->nfields) + (new_shape ? plan.work_bytes : 0);
+            if (e->used + need > e->budget && need <= e->budget) trim_pools(e);
+            if (e->used + need <= e->budget) {
+                if (new_shape) {
+                    Shape s;
+                    s.plan = plan;
+                    sit = e->shapes.emplace(key, s).first;
+                    reserved_work = plan.work_bytes;
+                }
+                sit->second.nruns++;  // the shape stays while this run is built
+                reserved = need;
+                e->used += need;
+                e->peak = std::max(e->peak, e->used);
+                e->admit_waits += waited;
+                break;
+            }
+            bool refuse = need > e->budget || wait_s <= 0;
+            if (!refuse) {
+                waited = true;
+                refuse = e->cv.wait_until(lk, deadline) == std::cv_status::timeout && e->used + need > e->budget;
+            }
+            if (refuse)
+                return set_error(VFD_ERR_BUDGET, "run_create: %zu bytes needed, %zu of %zu in use", need, e->used,
+                                 e->budget);
         }
-        need = run_bytes_for(plan, d->nfields) + (new_shape ? plan.work_bytes : 0);
-        if (e->used + need <= e->budget) break;
-        if (need > e->budget || wait_s <= 0 ||
-            (e->cv.wait_until(lk, deadline) == std::cv_status::timeout && e->used + need > e->budget))
-            return set_error(VFD_ERR_BUDGET, "run_create: %zu bytes needed, %zu of %zu in use", need, e->used,
-                             e->budget);
     }
     Run r;
     r.key = key;
@@ -445,36 +515,39 @@ static int run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id, double 
         }
         if (!rc) rc = plan_gauss_spectrum(plan, d->sigma2[f], d->decay[f], kp, nullptr);
     }
-    if (rc) {
-        for (void* p : r.field) (void)hipFree(p);
-        for (void* p : r.kspec) (void)hipFree(p);
-        return rc;
-    }
-    e->used += r.bytes;
-    if (new_shape) {
-        Shape s;
-        s.plan = plan;
-        // schedule for this shape, tuned once on scratch buffers when they fit
-        const int nf = std::min(8, d->nfields);
-        if (!getenv("VFD_NO_AUTOTUNE") &&
-            e->used + (size_t)nf * (plan.work_bytes + plan.field_bytes + plan.kspec_bytes) <= e->budget)
-            (void)plan_autotune(s.plan, nf, nullptr, nullptr);
-        sit = e->shapes.emplace(key, s).first;
-        Held h;
-        if (acquire(e, sit->second, 1, &h) < 1) {  // every shape keeps one work buffer
-            for (void* p : r.field) (void)hipFree(p);
-            for (void* p : r.kspec) (void)hipFree(p);
-            e->used -= r.bytes;
-            e->shapes.erase(sit);
-            return set_error(VFD_ERR_BUDGET, "run_create: no room for a work buffer");
+    std::vector<void*> frees;
+    size_t unreserve = 0;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        Shape& s = e->shapes.find(key)->second;  // kept by the nruns count taken above
+        if (reserved_work) {
+            // schedule for a new shape, tuned once per process on scratch buffers
+            // when they fit (plan_init took a cached schedule: a shape whose runs
+            // all left and that comes back is not tuned again)
+            const int nf = std::min(8, d->nfields);
+            if (!rc && !getenv("VFD_NO_AUTOTUNE") && !tuned_known(s.plan) &&
+                e->used + (size_t)nf * (plan.work_bytes + plan.field_bytes + plan.kspec_bytes) <= e->budget)
+                (void)plan_autotune(s.plan, nf, nullptr, nullptr);
+            e->used -= reserved_work;  // the shape's first work buffer is charged by acquire
+            reserved -= reserved_work;
+            Held h;
+            if (!rc && acquire(e, s, 1, &h) < 1)  // every shape keeps one work buffer
+                rc = set_error(VFD_ERR_BUDGET, "run_create: no room for a work buffer");
+            release(h);
         }
-        release(h);
+        if (!rc) {
+            const int id = e->next_run++;
+            e->runs[id] = r;
+            *run_id = id;
+            return VFD_OK;
+        }
+        frees = r.field;
+        frees.insert(frees.end(), r.kspec.begin(), r.kspec.end());
+        unreserve = reserved;
+        leave_shape(e, key, &frees, &unreserve);
     }
-    sit->second.nruns++;
-    const int id = e->next_run++;
-    e->runs[id] = r;
-    *run_id = id;
-    return VFD_OK;
+    free_and_return(e, frees, unreserve);
+    return rc;
 }
 
 int vfd_run_create(vfd_engine* e, const vfd_run_desc* d, int* run_id) { return run_create(e, d, run_id, 0.0); }
@@ -485,32 +558,24 @@ int vfd_run_create_wait(vfd_engine* e, const vfd_run_desc* d, double timeout_s, 
 
 int vfd_run_destroy(vfd_engine* e, int run_id) {
     if (!e) return set_error(VFD_ERR_INVALID, "run_destroy: null engine");
-    std::lock_guard<std::mutex> lk(e->mu);
-    auto it = e->runs.find(run_id);
-    if (it == e->runs.end()) return set_error(VFD_ERR_NOTFOUND, "run_destroy: unknown run %d", run_id);
-    if (it->second.busy) return set_error(VFD_ERR_STATE, "run_destroy: run %d has a tick in flight", run_id);
-    for (const Request& r : e->pending)
-        if (r.run == run_id) return set_error(VFD_ERR_STATE, "run_destroy: run %d has pending requests", run_id);
-    drop_graphs_with(e, run_id);
-    (void)hipDeviceSynchronize();
-    for (void* p : it->second.field) (void)hipFree(p);
-    for (void* p : it->second.kspec) (void)hipFree(p);
-    e->used -= it->second.bytes;
-    auto sit = e->shapes.find(it->second.key);
-    if (sit != e->shapes.end() && --sit->second.nruns == 0) {
-        // last run of this shape: its work buffers go back to the budget
-        bool held = false;
-        for (char c : sit->second.held) held |= c != 0;
-        if (!held) {
-            for (void* p : sit->second.pool) {
-                (void)hipFree(p);
-                e->used -= sit->second.plan.work_bytes;
-            }
-            e->shapes.erase(sit);
-        }
+    std::vector<void*> frees;
+    size_t bytes = 0;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        auto it = e->runs.find(run_id);
+        if (it == e->runs.end()) return set_error(VFD_ERR_NOTFOUND, "run_destroy: unknown run %d", run_id);
+        if (it->second.busy) return set_error(VFD_ERR_STATE, "run_destroy: run %d has a tick in flight", run_id);
+        for (const Request& r : e->pending)
+            if (r.run == run_id) return set_error(VFD_ERR_STATE, "run_destroy: run %d has pending requests", run_id);
+        drop_graphs_with(e, run_id);
+        const Run& r = it->second;
+        frees = r.field;
+        frees.insert(frees.end(), r.kspec.begin(), r.kspec.end());
+        bytes = r.bytes;
+        leave_shape(e, r.key, &frees, &bytes);
+        e->runs.erase(it);
     }
-    e->runs.erase(it);
-    e->cv.notify_all();  // admissions waiting for memory (vfd_run_create_wait) recheck
+    free_and_return(e, frees, bytes);  // admissions waiting for memory recheck
     return VFD_OK;
 }
 
@@ -716,6 +781,8 @@ int vfd_engine_get_stats(vfd_engine* e, vfd_engine_stats* s) {
     s->ticks = e->ticks;
     s->requests_done = e->done_count;
     s->requests_pending = e->pending.size();
+    s->peak_used_bytes = e->peak;
+    s->runs_admitted_after_wait = e->admit_waits;
     return VFD_OK;
 }
 

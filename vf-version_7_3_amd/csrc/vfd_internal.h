@@ -35,6 +35,7 @@ int twiddles(int p, int prec, const void** out);
 
 int plan_init(const vfd_plan_desc* d, Plan* p);
 void tuned_lookup(Plan& q);
+bool tuned_known(const Plan& q);  // a tuned schedule of this shape is cached
 int plan_autotune(Plan& p, int nfields, hipStream_t st, float* best_ms);
 int plan_diffuse(const Plan& p, const vfd_field* fields, int nfields, hipStream_t st);
 float chem_display_scale(int chem);  // sampleChem's per-chem display scale, chem < 8

@@ -123,6 +123,11 @@ void tuned_lookup(Plan& q) {
         for (int i = 0; i < 5; ++i) q.var[i] = it->second[i];
 }
 
+bool tuned_known(const Plan& q) {
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    return g_sched.count(shape_key(q)) != 0;
+}
+
 // ------------------------------------------------------------------ plans
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
 

@@ -19,7 +19,8 @@ class RunDesc(ctypes.Structure):
 
 class EngineStats(ctypes.Structure):
     _fields_ = [("budget_bytes", _sz), ("used_bytes", _sz), ("runs", _i), ("shapes", _i),
-                ("batches", _u64), ("ticks", _u64), ("requests_done", _u64), ("requests_pending", _u64)]
+                ("batches", _u64), ("ticks", _u64), ("requests_done", _u64), ("requests_pending", _u64),
+                ("peak_used_bytes", _sz), ("runs_admitted_after_wait", _u64)]
 
 
 lib.vfd_engine_create.argtypes = [_sz, ctypes.POINTER(_p)]
