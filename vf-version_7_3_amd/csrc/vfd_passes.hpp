@@ -14,6 +14,8 @@
 // R2C (Diffusion.cu:1814), modulateAndNormalize (convolutionFFT.cuh:690-716),
 // C2R (Diffusion.cu:1842), unpadResult3D (convolutionFFT.cuh:152-171).
 #pragma once
+#include <type_traits>
+
 #include "vfd_fft.hpp"
 #include "vfd_types.h"
 
@@ -443,8 +445,11 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     // narrower tile (long columns) shares each line with its neighbours, which
     // must still find it in the cache
     constexpr int NTP = C::TC * (int)CB >= 128 ? kStream : kCacheDefault;
-    // stores: non-temporal down to 64-byte row segments; a narrower segment is
-    // written back through the L2, where the tiles sharing its line merge
+    // stores: non-temporal for whole 128-byte row segments; a narrower segment
+    // is written back through the L2, where the tiles sharing its line merge.
+    // (Choosing the L2 path at run time for the last, mostly empty tile of a
+    // 1025-column row measured 10-30 % slower on the y forward and z FFT
+    // passes; the direct z pass, one column per lane, gains from it.)
     constexpr int NTS = C::TC * (int)CB >= 128 ? kStream : kCacheDefault;
     constexpr int NI = ShF::NB0 * ShF::R0;
     constexpr int NO = ShI::NBL * ShI::RL;
@@ -652,8 +657,9 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
     if (zform_of_table(fd.kspec, cd.kspec_bytes, cd.sep_g) != RM) return;
     const auto rwork = B::make(fd.work, cd.work_bytes);
     const long col = (long)blockIdx.x * 64 + threadIdx.x;
+    const long ky = blockIdx.y;
     const bool live = col < cd.ncols;
-    const O base = (O)(((long)blockIdx.y * cd.pstride + (live ? col : 0)) * CB);
+    const O base = (O)((live ? ky * cd.pstride + col : 0) * CB);
     const O rstep = (O)cd.rstride * CB;
     const int nz = cd.nin;
     const auto tk = BD::make(fd.kspec, cd.kspec_bytes);
@@ -661,7 +667,7 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
 #pragma unroll
     for (int j = 0; j <= RM; ++j) g[j] = (float)BD::ld_r(tk, (OD)(cd.sep_g + 1 + j) * 8u);
     const double sc = BD::ld_r(tk, OD(0));
-    const double gyv = BD::ld_r(tk, (OD)(cd.sep_gy + blockIdx.y) * 8u);
+    const double gyv = BD::ld_r(tk, live ? (OD)(cd.sep_gy + ky) * 8u : BD::OOB);
     const double gxv = BD::ld_r(tk, live ? (OD)(1 + col) * 8u : BD::OOB);
     // The taps as rounded to FP32 sum (exactly) to gs rather than 1: the column
     // factor takes up the difference, so the window conserves mass
@@ -671,29 +677,39 @@ zdirect_kernel(const FieldBatch fb, ColDesc cd) {
     const double kad = (((gxv * gyv) * sc) * cd.scale) / gs;  // cd.scale = Pz here
     const float ka = (float)kad, kl = (float)(kad - (double)ka);
     auto row = [&](int m) -> O { return (live && m >= 0 && m < nz) ? base + (O)m * rstep : B::OOB; };
-    // w[(n - RM + k) mod U] holds x[n - RM + k]; x[-RM .. RM] to start
-    cx<float> w[U];
+    auto body = [&](auto pol) {
+        constexpr int A = decltype(pol)::value;
+        // w[(n - RM + k) mod U] holds x[n - RM + k]; x[-RM .. RM] to start
+        cx<float> w[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k) w[k] = B::ld_cx(rwork, row(k - RM));
-    for (int n0 = 0; n0 < nz; n0 += U) {
+        for (int k = 0; k < U; ++k) w[k] = B::template ld_cx<A>(rwork, row(k - RM));
+        for (int n0 = 0; n0 < nz; n0 += U) {
 #pragma unroll
-        for (int i = 0; i < U; ++i) {
-            // output n = n0 + i; its centre x[n] sits at slot (i + RM) mod U
-            const int n = n0 + i;
-            const int c = (i + RM) % U;
-            cx<float> acc{g[0] * w[c].x, g[0] * w[c].y};
+            for (int i = 0; i < U; ++i) {
+                // output n = n0 + i; its centre x[n] sits at slot (i + RM) mod U
+                const int n = n0 + i;
+                const int c = (i + RM) % U;
+                cx<float> acc{g[0] * w[c].x, g[0] * w[c].y};
 #pragma unroll
-            for (int j = 1; j <= RM; ++j) {
-                const cx<float> a = w[(c + U - j) % U], b = w[(c + j) % U];
-                acc.x = fmaf(g[j], a.x + b.x, acc.x);
-                acc.y = fmaf(g[j], a.y + b.y, acc.y);
+                for (int j = 1; j <= RM; ++j) {
+                    const cx<float> a = w[(c + U - j) % U], b = w[(c + j) % U];
+                    acc.x = fmaf(g[j], a.x + b.x, acc.x);
+                    acc.y = fmaf(g[j], a.y + b.y, acc.y);
+                }
+                // x[n - RM] is no longer needed: its slot takes x[n + RM + 1]
+                w[i] = B::template ld_cx<A>(rwork, row(n + RM + 1));
+                B::template st_cx<A>(cx<float>{fmaf(ka, acc.x, kl * acc.x), fmaf(ka, acc.y, kl * acc.y)}, rwork,
+                                     row(n));
             }
-            // x[n - RM] is no longer needed: its slot takes x[n + RM + 1]
-            w[i] = B::ld_cx(rwork, row(n + RM + 1));
-            B::template st_cx<kStream>(cx<float>{fmaf(ka, acc.x, kl * acc.x), fmaf(ka, acc.y, kl * acc.y)}, rwork,
-                                       row(n));
         }
-    }
+    };
+    // Each element is read once and written once: non-temporal both ways for
+    // a wave's full 512-byte row segments.  A work-group with part of its 64
+    // columns past the row's end (the last one, 1025 = 16 * 64 + 1 columns on
+    // a 2048 box) goes through the L2: a narrow non-temporal store costs a
+    // partial-line write to HBM (this form is 8 % faster at radius 16).
+    if ((long)(blockIdx.x + 1) * 64 <= cd.ncols) body(std::integral_constant<int, kStream>{});
+    else body(std::integral_constant<int, kCacheDefault>{});
 }
 
 }  // namespace vfd
