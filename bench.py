@@ -163,6 +163,7 @@ def bench_copy(args):
                   {"workload": "vfd_stream_copy of 1 GiB (16-byte non-temporal loads/stores, one per lane)"})
     out["frac_of_spec"] = gbs / HBM_PEAK_GBS
     out["torch_copy_gbs"] = 2.0 * n * 4 / (ms_torch * 1e-3) / 1e9
+    out["mix_gbs"] = mix_bandwidth(vfd, torch, stream)
     print(json.dumps(out), flush=True)
 
 
@@ -772,6 +773,40 @@ def copy_bandwidth(vfd, torch, stream, nbytes=1 << 30, reps=7):
     return 2.0 * nbytes / (ts[len(ts) // 2] * 1e-3) / 1e9
 
 
+def mix_bandwidth(vfd, torch, stream, nbytes=1 << 30, reps=7):
+    """GB/s (all bytes moved over time, median of reps) of the library's
+    streaming kernel for three read:write mixes: r1w1 (the copy), r1w2 (one
+    buffer read, two written: the mix of the x and y forward passes) and r2w1
+    (their inverses)."""
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b, c = torch.empty_like(a), torch.empty_like(a)
+    a.fill_(1)
+    b.fill_(2)
+    forms = {"r1w1": (lambda: vfd.stream_mix(c, a, stream=stream), 2),
+             "r1w2": (lambda: vfd.stream_mix(b, a, dst1=c, stream=stream), 3),
+             "r2w1": (lambda: vfd.stream_mix(c, a, src1=b, stream=stream), 3)}
+    out = {}
+    for name, (fn, nbuf) in forms.items():
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ts.sort()
+        out[name] = nbuf * nbytes / (ts[len(ts) // 2] * 1e-3) / 1e9
+    del a, b, c
+    return out
+
+
+# read:write mix of each launch of the step (x-first order)
+LAUNCH_MIX = {"xfwd": "r1w2", "yfwd": "r1w2", "z_direct8": "r1w1", "z_direct12": "r1w1", "z_direct16": "r1w1",
+              "z_fft": "r1w1", "yinv": "r2w1", "xinv": "r2w1"}
+
+
 def fft_z_step(vfd, torch, np, shape, fields, params, model_bytes, stream, warmup, steps):
     """The same step with the spectrum streamed as the 3D real array: every
     field takes the FFT form of the z pass (the form a non-separable kernel,
@@ -952,6 +987,7 @@ def main():
     step_bytes = plan.info.bytes_per_field_tick * nf
     mean_step_ms = float(np.mean(step_ms))
     copy_gbs = copy_bandwidth(vfd, torch, stream)
+    mix_gbs = mix_bandwidth(vfd, torch, stream)
     fft_z = None if args.no_fft_z else fft_z_step(vfd, torch, np, shape, fields, params, model_bytes, stream,
                                                   args.warmup, min(K, 20))
 
@@ -994,6 +1030,11 @@ def main():
             # measured streaming reference: 16-byte non-temporal copy of 1 GiB
             "copy_gbs": copy_gbs,
             "step_frac_of_copy": step_bytes / (mean_step_ms * 1e-3) / 1e9 / copy_gbs,
+            # the same streaming kernel for each launch's read:write mix, and
+            # each launch's GB/s as a fraction of its mix's
+            "mix_gbs": mix_gbs,
+            "kernels_frac_of_mix": {n: (g / mix_gbs[LAUNCH_MIX[n]] if g > 0 else None)
+                                    for (n, _), g in zip(launches, kern_gbs)},
             # the survey's byte model with a streamed real K^ (8N + 32 A1 + 32 A2 + 4 A3):
             # counts 4 A3 per field that the separable spectrum never reads
             "model_bytes_per_step": model_bytes,

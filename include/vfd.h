@@ -137,6 +137,11 @@ double vfd_plan_pass_bytes(const vfd_plan* plan, int pass);
  * 16-byte non-temporal load and store per lane: the streaming-bandwidth
  * reference the passes are measured against.  Asynchronous. */
 int vfd_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* The same streaming form for other read:write mixes, the references for
+ * passes that read one byte per two written (dst1 given: dst0 = dst1 = src0)
+ * or two per one (src1 given: dst0 = src0 + src1 as 32-bit integer lanes).
+ * With neither it is vfd_stream_copy.  Buffers of `bytes` each.  Asynchronous. */
+int vfd_stream_mix(void* dst0, void* dst1, const void* src0, const void* src1, size_t bytes, void* stream);
 
 /* Schedule autotuner: every compiled schedule variant (radix 16/8, work-group
  * size) of every pass is timed on scratch buffers for nfields fields; the

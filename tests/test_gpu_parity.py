@@ -372,3 +372,28 @@ def test_separable_table_at_reused_address(vfd, orc):
     torch.cuda.synchronize()
     for i, (s2, dec) in enumerate((small, large)):
         assert _report(f"mixed batch {i}", outs[i].cpu().numpy(), orc.direct_separable(u, s2, dec)) <= REL_F32
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 48, (1 << 20) + 16 * 37])
+def test_stream_reference_kernels(vfd, nbytes):
+    """The streaming reference kernels move exactly the bytes asked for: the
+    copy, the one-read-two-writes form and the two-reads-one-write form
+    (32-bit integer lanes added), on sizes that end inside a work-group; the
+    bytes after the end stay untouched."""
+    import torch
+    print(f"case stream_reference_kernels nbytes={nbytes}", flush=True)
+    n = nbytes // 4
+    g = torch.Generator(device="cpu").manual_seed(nbytes)
+    a = torch.randint(0, 1 << 30, (n + 64,), generator=g, dtype=torch.int32).cuda()
+    b = torch.randint(0, 1 << 30, (n + 64,), generator=g, dtype=torch.int32).cuda()
+    c = torch.full((n + 64,), -7, dtype=torch.int32, device="cuda")
+    d = torch.full((n + 64,), -9, dtype=torch.int32, device="cuda")
+    vfd.stream_copy(c[:n], a[:n])
+    torch.cuda.synchronize()
+    assert torch.equal(c[:n], a[:n]) and bool((c[n:] == -7).all())
+    vfd.stream_mix(c[:n], a[:n], dst1=d[:n])
+    torch.cuda.synchronize()
+    assert torch.equal(c[:n], a[:n]) and torch.equal(d[:n], a[:n]) and bool((d[n:] == -9).all())
+    vfd.stream_mix(c[:n], a[:n], src1=b[:n])
+    torch.cuda.synchronize()
+    assert torch.equal(c[:n], a[:n] + b[:n]) and bool((c[n:] == -7).all())

@@ -826,6 +826,22 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const void* src, void*
                                            kStream);
 }
 
+// The same form with a second source (added) or a second destination: the
+// streaming references for passes that read one byte per two written (x and
+// y forward) or two per one (their inverses).
+template <bool R2, bool W2>
+__global__ void __launch_bounds__(256) stream_mix_kernel(const void* src0, const void* src1, void* dst0, void* dst1,
+                                                         unsigned nvec) {
+    const unsigned i = blockIdx.x * 256u + threadIdx.x;
+    const unsigned off = i < nvec ? i * 16u : kBufOOB;
+    u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(make_rsrc(src0, (unsigned long)nvec * 16u), off, 0, kStream);
+    if constexpr (R2) v += __builtin_amdgcn_raw_buffer_load_b128(make_rsrc(src1, (unsigned long)nvec * 16u), off, 0,
+                                                                 kStream);
+    __builtin_amdgcn_raw_buffer_store_b128(v, make_rsrc(dst0, (unsigned long)nvec * 16u), off, 0, kStream);
+    if constexpr (W2) __builtin_amdgcn_raw_buffer_store_b128(v, make_rsrc(dst1, (unsigned long)nvec * 16u), off, 0,
+                                                             kStream);
+}
+
 bool length_supported(int q) {
     switch (q) {
 #define VFD_CASE(Q) case Q:
@@ -927,6 +943,21 @@ int vfd_diffuse_pass(const vfd_plan* plan, const vfd_field* fields, int nfields,
     if (pass >= VFD_PASS_Z_DIRECT8 && pass <= VFD_PASS_Z_FFT)
         return plan_diffuse_zpart(plan->p, fields, nfields, pass - VFD_PASS_Z_DIRECT8, (hipStream_t)stream);
     return plan_diffuse_passes(plan->p, fields, nfields, pass, pass + 1, (hipStream_t)stream);
+}
+
+int vfd_stream_mix(void* dst0, void* dst1, const void* src0, const void* src1, size_t bytes, void* stream) {
+    if (!dst0 || !src0) return set_error(VFD_ERR_INVALID, "stream_mix: null buffer");
+    if (bytes % 16 || bytes >= kBufLimit)
+        return set_error(VFD_ERR_INVALID, "stream_mix: %zu bytes (a multiple of 16 below 4 GiB)", bytes);
+    if (bytes == 0) return VFD_OK;
+    const unsigned nvec = (unsigned)(bytes / 16);
+    const dim3 grid((nvec + 255) / 256);
+    hipStream_t st = (hipStream_t)stream;
+    if (src1 && dst1) hipLaunchKernelGGL((stream_mix_kernel<true, true>), grid, dim3(256), 0, st, src0, src1, dst0, dst1, nvec);
+    else if (src1) hipLaunchKernelGGL((stream_mix_kernel<true, false>), grid, dim3(256), 0, st, src0, src1, dst0, dst0, nvec);
+    else if (dst1) hipLaunchKernelGGL((stream_mix_kernel<false, true>), grid, dim3(256), 0, st, src0, src0, dst0, dst1, nvec);
+    else hipLaunchKernelGGL((stream_mix_kernel<false, false>), grid, dim3(256), 0, st, src0, src0, dst0, dst0, nvec);
+    return hip_check(hipGetLastError(), "stream_mix launch");
 }
 
 int vfd_stream_copy(void* dst, const void* src, size_t bytes, void* stream) {

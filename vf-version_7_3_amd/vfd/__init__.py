@@ -88,6 +88,7 @@ lib.vfd_chem_update.argtypes = [ctypes.POINTER(_p), ctypes.POINTER(_p), ctypes.P
                                 ctypes.POINTER(_d), _p]
 lib.vfd_c2r_3d.argtypes = [_p, _p, _p, _p]
 lib.vfd_stream_copy.argtypes = [_p, _p, _sz, _p]
+lib.vfd_stream_mix.argtypes = [_p, _p, _p, _p, _sz, _p]
 
 
 def stream_copy(dst, src, stream=None):
@@ -97,6 +98,18 @@ def stream_copy(dst, src, stream=None):
     if dst.numel() * dst.element_size() != nbytes:
         raise ValueError("stream_copy: tensors differ in size")
     check(lib.vfd_stream_copy(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
+
+
+def stream_mix(dst0, src0, dst1=None, src1=None, stream=None):
+    """The streaming kernel for other read:write mixes: dst1 given, src0 is
+    written to dst0 and dst1 (1 read : 2 writes); src1 given, dst0 = src0 +
+    src1 as 32-bit integer lanes (2 reads : 1 write).  Equal byte sizes."""
+    nbytes = src0.numel() * src0.element_size()
+    for t in (dst0, dst1, src1):
+        if t is not None and t.numel() * t.element_size() != nbytes:
+            raise ValueError("stream_mix: tensors differ in size")
+    check(lib.vfd_stream_mix(_ptr(dst0), None if dst1 is None else _ptr(dst1), _ptr(src0),
+                             None if src1 is None else _ptr(src1), nbytes, _stream(stream)))
 
 
 def check(rc):
