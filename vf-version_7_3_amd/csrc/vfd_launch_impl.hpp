@@ -18,16 +18,12 @@
 
 namespace vfd {
 
-// resident work-groups per CU for a kernel (cached per instantiation)
+// resident work-groups per CU for a kernel (callers cache it per instantiation)
 template <typename K>
 static int blocks_per_cu(K kernel, int nt) {
-    static int cached = 0;
-    if (!cached) {
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, nt, 0) != hipSuccess || n <= 0) n = 1;
-        cached = n;
-    }
-    return cached;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, nt, 0) != hipSuccess || n <= 0) n = 1;
+    return n;
 }
 
 [[maybe_unused]] static int num_cus() {
@@ -53,7 +49,8 @@ static hipError_t row_go(const FieldBatch& fields, int nfields, const GridDesc& 
     // (wave-level stage sync) balance better over the CUs than with one round
     // (x passes 1.34 -> 1.31 ms at 1024x512x128, 8 fields), while a work-group
     // per row group restages the twiddle table for every 4 rows (1.65 ms)
-    const long cap = 4L * num_cus() * blocks_per_cu(kern, C::NT);
+    static const int per_cu = blocks_per_cu(kern, C::NT);  // this instantiation's
+    const long cap = 4L * num_cus() * per_cu;
     const unsigned grid = (unsigned)(items < cap ? items : cap);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(C::NT), 0, st, fields, g, (const cx<T>*)tw, nfields);
     return hipGetLastError();
