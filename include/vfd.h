@@ -83,6 +83,8 @@ typedef struct {
     size_t kspec_bytes; /* per-field kernel spectrum in the plan's format */
     double bytes_per_field_tick; /* algorithmic HBM bytes of one tick (pruned model) */
     int order;          /* pass order, VFD_ORDER_XFIRST or VFD_ORDER_XLAST */
+    int y_inv_fold;     /* 1: the inverse y pass splits its py-long columns even/odd
+                           with the data rows past py/2 folded in (FP32, ny > py/2) */
 } vfd_plan_info;
 
 /* Pass orders.  XFIRST: x R2C, y, z (fused forward * K^ * inverse), y inverse,

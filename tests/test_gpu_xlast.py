@@ -46,6 +46,8 @@ def _spectrum(orc, box, key, symmetric=False):
     ((100, 60, 21), (128, 72, 64), 3, 30, 1.5),     # epiBoundary beyond the row: bounded at nx
     ((25, 30, 40), (32, 40, 192), 3, 0, 0.5),       # no interior: the whole box is baseChem
     ((30, 17, 100), (32, 18, 256), 3, 97, 0.2),     # radix-3 y box
+    ((20, 600, 6), (32, 1024, 32), 3, 15, 0.4),     # y rows past P/2: folded split, Q = 512
+    ((12, 800, 5), (32, 1536, 32), 100, 0, 0.0),    # folded split, Q = 768, zero pad
 ])
 def test_xlast_matches_oracle_and_xfirst(vfd, orc, monkeypatch, shape, box, pad, epi, base):
     import torch
@@ -131,3 +133,17 @@ def test_xlast_fused_update_and_sample_bit_identical(vfd, orc, monkeypatch, shap
     for k in range(3):
         assert torch.equal(out[k], t[k]), k
         assert torch.equal(vis[k], vfd.sample_chem(t[k], chems[k], stride)), k
+
+
+def test_y_fold_selection(vfd):
+    """The FP32 inverse y pass of columns longer than half their box takes
+    the folded even/odd split where it is compiled (P/2 = 512 or 768), the
+    ABM's production box among them; FP64 plans and columns within half their
+    box keep the plain split."""
+    print("case y_fold_selection", flush=True)
+    box = (32, 1024, 32)
+    assert vfd.Plan((20, 600, 6), box, vfd.F32, vfd.PAD_ZERO, vfd.KSPEC_COMPLEX_STD).info.y_inv_fold == 1
+    assert vfd.Plan((20, 600, 6), box, vfd.F64, vfd.PAD_ZERO, vfd.KSPEC_COMPLEX_STD).info.y_inv_fold == 0
+    assert vfd.Plan((20, 500, 6), box, vfd.F32, vfd.PAD_ZERO, vfd.KSPEC_COMPLEX_STD).info.y_inv_fold == 0
+    prod = vfd.Plan((1006, 1390, 110), (1152, 1536, 256), vfd.F32, vfd.PAD_CONSTANT_VF, vfd.KSPEC_COMPLEX_STD)
+    assert prod.info.y_inv_fold == 1

@@ -14,6 +14,7 @@ struct Plan {
     int prec, pad_mode, kspec_format;
     int L, hx;
     int rs_y, rs_z, qy, qz;
+    int fold_y;  // y inverse pass split and folded (ny > py/2, col_kernel FOLD with rs = 2)
     bool full_box;
     long ld, plane;
     long ks_row, ks_plane;

@@ -7,6 +7,9 @@ namespace vfd {
 
 // True when a transform of this length is compiled into the library.
 bool length_supported(int q);
+// Sub-lengths Q (= P/2) with the folded even/odd split of the FP32 inverse y
+// pass compiled (col_kernel FOLD): long columns whose unsplit tiles are narrow.
+constexpr bool fold_supported(int q) { return q == 512 || q == 768; }
 
 // x-forward (pad + R2C) over Ny*Nz rows of nfields fields; L = Px/2.
 // `variant` selects a compiled schedule (0..3, see vfd_passes.hpp); lengths

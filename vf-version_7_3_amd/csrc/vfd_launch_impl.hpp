@@ -122,7 +122,7 @@ hipError_t launch_xinv<VFD_REAL>(int L, const FieldBatch& fields, int nfields, c
 #endif
 
 #ifdef VFD_COL_TU
-template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
+template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL, bool FOLD = false>
 static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                          const void* tw, hipStream_t st) {
     using C = ColCfg<T, Q, V, RS>;
@@ -135,11 +135,11 @@ static hipError_t col_go(const FieldBatch& fields, int nfields, const ColDesc& c
         const unsigned long groups = (unsigned long)g.grid_segs * nplanes * nfields;
         const unsigned long ids = (groups + 7) / 8 * 8 * C::SH;
         if (ids > 0x7fffffffu) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), dim3((unsigned)ids), dim3(C::NT), 0, st, fields, g,
-                           (const cx<T>*)tw);
+        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS, FOLD>), dim3((unsigned)ids), dim3(C::NT), 0, st, fields,
+                           g, (const cx<T>*)tw);
     } else {
         dim3 grid((unsigned)((cd.ncols + C::TC - 1) / C::TC), (unsigned)nplanes, (unsigned)nfields);
-        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS>), grid, dim3(C::NT), 0, st, fields, cd,
+        hipLaunchKernelGGL((col_kernel<T, Q, MODE, RS, V, KS, FOLD>), grid, dim3(C::NT), 0, st, fields, cd,
                            (const cx<T>*)tw);
     }
     return hipGetLastError();
@@ -157,6 +157,13 @@ static hipError_t col_fused(const FieldBatch& fields, int nfields, const ColDesc
 template <typename T, int Q, int V>
 static hipError_t col_mode(const FieldBatch& fields, int nfields, const ColDesc& cd, long nplanes,
                            const void* tw, hipStream_t st) {
+    if (cd.fold) {
+        if constexpr (sizeof(T) == 4 && fold_supported(Q)) {
+            if (cd.rs == 2 && cd.mode == COL_INV)
+                return col_go<T, Q, COL_INV, 2, V, KS_REAL, true>(fields, nfields, cd, nplanes, tw, st);
+        }
+        return hipErrorInvalidValue;
+    }
     const int key = cd.mode * 2 + (cd.rs - 1);
     switch (key) {
         case COL_FWD * 2 + 0: return col_go<T, Q, COL_FWD, 1, V>(fields, nfields, cd, nplanes, tw, st);

@@ -396,10 +396,17 @@ template <typename T, int MODE, int NT> constexpr int col_waves_per_eu() {
     return sizeof(T) == 4 && MODE == COL_INV && NT == 512 ? 4 : 1;
 }
 
-template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL>
+// FOLD (RS = 2, inverse-only mode): the output rows reach past Q = P/2 (a box
+// less than twice the data, as the ABM's 1536 rows for 1390 cells), so the
+// even/odd split folds them: with Y_s the inverse transform of the sub-sequence
+// X[2k + s], row n takes Y_0[n] + W_P^-n Y_1[n] and row n + Q takes
+// Y_0[n] - W_P^-n Y_1[n].  Half the LDS per column of the unsplit transform:
+// twice the columns per tile.
+template <typename T, int Q, int MODE, int RS, int V, int KS = KS_REAL, bool FOLD = false>
 __global__ void __launch_bounds__((ColCfg<T, Q, V, RS>::NT))
 __attribute__((amdgpu_waves_per_eu((col_waves_per_eu<T, MODE, ColCfg<T, Q, V, RS>::NT>()))))
 col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
+    static_assert(!FOLD || (RS == 2 && MODE == COL_INV), "folded split: RS = 2, inverse-only mode");
     using C = ColCfg<T, Q, V, RS>;
     using ShF = FftShape<T, Q, C::TPF, false, C::MAXR>;                // forward / inverse-only plan
     using ShI = FftShape<T, Q, C::TPF, MODE == COL_FUSED, C::MAXR>;    // inverse plan
@@ -458,6 +465,7 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     constexpr int NII = ShI::NB0 * ShI::R0;
     cx<T> x[MODE == COL_INV ? NII * RS : NI];   // saved inputs (all sub-sequences in INV mode)
     cx<T> acc[MODE == COL_FWD ? 1 : NO]; // accumulated outputs (inverse modes)
+    [[maybe_unused]] cx<T> acq[FOLD && MODE == COL_INV ? NO : 1];  // folded: output rows n + Q
     cx<T> v[ShF::E > ShI::E ? ShF::E : ShI::E];
 
     if constexpr (MODE != COL_INV) {
@@ -486,6 +494,10 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
     if constexpr (MODE != COL_FWD) {
 #pragma unroll
         for (int i = 0; i < NO; ++i) acc[i] = cx<T>{T(0), T(0)};
+        if constexpr (FOLD) {
+#pragma unroll
+            for (int i = 0; i < NO; ++i) acq[i] = cx<T>{T(0), T(0)};
+        }
     }
 
     // kernel spectrum for the fused mode, prefetched before any transform work:
@@ -611,6 +623,9 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                     cx<T> val = v[b * ShI::RL + r];
                     if (s) val = cmulc(val, tw[n >= 0 ? n : 0]);  // W_P^-n
                     acc[b * ShI::RL + r] = cadd(acc[b * ShI::RL + r], val);
+                    // row n + Q: W_P^-(n+Q)s = (-1)^s W_P^-ns
+                    if constexpr (FOLD && MODE == COL_INV)
+                        acq[b * ShI::RL + r] = s ? csub(acq[b * ShI::RL + r], val) : cadd(acq[b * ShI::RL + r], val);
                 }
         }
     }
@@ -622,6 +637,10 @@ col_kernel(const FieldBatch fb, ColDesc cd, const cx<T>* __restrict__ twg) {
                 const int n = ShI::out_elem(t, b, r);
                 const bool ok = n >= 0 && n < cd.nout && live;
                 B::template st_cx<NTS>(acc[b * ShI::RL + r], rwork, ok ? base + (O)n * rstep : B::OOB);
+                if constexpr (FOLD && MODE == COL_INV) {
+                    const bool okq = n >= 0 && n + Q < cd.nout && live;
+                    B::template st_cx<NTS>(acq[b * ShI::RL + r], rwork, okq ? base + (O)(n + Q) * rstep : B::OOB);
+                }
             }
     }
 }

@@ -188,6 +188,13 @@ int plan_init(const vfd_plan_desc* d, Plan* p) {
     };
     if (!pick(q.ny, q.py, &q.rs_y, &q.qy))
         return set_error(VFD_ERR_UNSUPPORTED, "plan: y transform length %d not compiled in", q.py);
+    // FP32 y columns longer than half their box (the ABM's 1390 rows on 1536):
+    // the inverse y pass takes the folded even/odd split where it is compiled,
+    // half the LDS per column and twice the columns per tile (production box:
+    // 0.38 -> 0.35 ms per field).  The forward pass keeps the unsplit
+    // transform: folded, its two half-passes write alternate rows of 64-byte
+    // segments and it ran slower (0.40 -> 0.53 ms).
+    q.fold_y = q.prec == VFD_F32 && q.rs_y == 1 && q.py % 2 == 0 && q.ny > q.py / 2 && fold_supported(q.py / 2);
     if (!pick(q.nz, q.pz, &q.rs_z, &q.qz))
         return set_error(VFD_ERR_UNSUPPORTED, "plan: z transform length %d not compiled in", q.pz);
     q.full_box = (q.nx == q.px && q.ny == q.py && q.nz == q.pz);
@@ -322,13 +329,15 @@ static int run_xlast(const Plan& p, const FieldBatch& fb, int nf, int pass_lo, i
     y.work_bytes = p.work_bytes; y.kspec_bytes = p.kspec_bytes;
     ColDesc yi = y;
     yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
+    if (p.fold_y) { yi.rs = 2; yi.fold = 1; }
+    const int qyi = p.fold_y ? p.py / 2 : p.qy;
     for (int pass = pass_lo; pass < pass_hi; ++pass) {
         hipError_t e = hipSuccess;
         switch (pass) {
             case VFD_PASS_XFWD: e = launch_zr(true, p.pz, fb, nf, d, p.tw_z, st, p.var[pass]); break;
             case VFD_PASS_YFWD: e = launch_col<float>(p.qy, fb, nf, y, p.hz, p.tw_y, st, p.var[pass]); break;
             case VFD_PASS_ZCONV: e = launch_xc(p.px, fb, nf, d, p.tw_x, st, p.var[pass]); break;
-            case VFD_PASS_YINV: e = launch_col<float>(p.qy, fb, nf, yi, p.hz, p.tw_y, st, p.var[pass]); break;
+            case VFD_PASS_YINV: e = launch_col<float>(qyi, fb, nf, yi, p.hz, p.tw_y, st, p.var[pass]); break;
             case VFD_PASS_XINV: e = launch_zr(false, p.pz, fb, nf, d, p.tw_z, st, p.var[pass]); break;
         }
         if (e != hipSuccess) return hip_check(e, "x-last pass launch");
@@ -381,6 +390,8 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
         z.work_bytes = p.work_bytes; z.kspec_bytes = p.kspec_bytes;
         ColDesc yi = y;
         yi.mode = COL_INV; yi.nin = p.py; yi.nout = p.ny;
+        if (p.fold_y) { yi.rs = 2; yi.fold = 1; }
+        const int qyi = p.fold_y ? p.py / 2 : p.qy;
         if constexpr (sizeof(T) == 4) {
             if (p.order == VFD_ORDER_XLAST) {
                 if (int rc = run_xlast(p, fb, nf, pass_lo, pass_hi, st)) return rc;
@@ -393,7 +404,7 @@ static int run_tick(const Plan& p, const vfd_field* fields, int nfields, int pas
                 case VFD_PASS_XFWD: e = launch_xfwd<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass]); break;
                 case VFD_PASS_YFWD: e = launch_col<T>(p.qy, fb, nf, y, p.nz, p.tw_y, st, p.var[pass]); break;
                 case VFD_PASS_ZCONV: e = launch_z<T>(p, fb, nf, z, st, zpart); break;
-                case VFD_PASS_YINV: e = launch_col<T>(p.qy, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
+                case VFD_PASS_YINV: e = launch_col<T>(qyi, fb, nf, yi, p.nz, p.tw_y, st, p.var[pass]); break;
                 case VFD_PASS_XINV:
                     e = launch_xinv<T>(p.L, fb, nf, g, p.tw_x, st, p.var[pass], epi && epi->vis);
                     break;
@@ -888,6 +899,7 @@ int vfd_plan_get_info(const vfd_plan* plan, vfd_plan_info* info) {
     info->pz = p.pz; info->py = p.py; info->px = p.px;
     info->L = p.L; info->hx = p.hx; info->ld = p.ld;
     info->rs_y = p.rs_y; info->rs_z = p.rs_z;
+    info->y_inv_fold = p.fold_y;
     info->work_bytes = p.work_bytes;
     info->kspec_bytes = p.kspec_bytes;
     info->bytes_per_field_tick = p.bytes_per_field_tick;

@@ -67,6 +67,7 @@ struct ColDesc {
     // fields of the launch, 128-byte row segments per plane row
     unsigned grid_planes, grid_fields, grid_segs;
     int sep_g;          // separable: table offset (elements) of [R | g_0 .. g_kZDirectMaxR]
+    int fold;           // rs = 2 with output rows past P/2 (FP32 inverse-only, fold_supported)
 };
 
 // The x-last pass order (z R2C, y C2C, x fused, y inverse, z C2R) for boxes

@@ -52,7 +52,8 @@ class PlanDesc(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("nz", _i), ("ny", _i), ("nx", _i), ("pz", _i), ("py", _i), ("px", _i),
                 ("L", _i), ("hx", _i), ("ld", ctypes.c_long), ("rs_y", _i), ("rs_z", _i),
-                ("work_bytes", _sz), ("kspec_bytes", _sz), ("bytes_per_field_tick", _d), ("order", _i)]
+                ("work_bytes", _sz), ("kspec_bytes", _sz), ("bytes_per_field_tick", _d), ("order", _i),
+                ("y_inv_fold", _i)]
 
 
 class Field(ctypes.Structure):
