@@ -48,6 +48,8 @@ def _spectrum(orc, box, key, symmetric=False):
     ((30, 17, 100), (32, 18, 256), 3, 97, 0.2),     # radix-3 y box
     ((20, 600, 6), (32, 1024, 32), 3, 15, 0.4),     # y rows past P/2: folded split, Q = 512
     ((12, 800, 5), (32, 1536, 32), 100, 0, 0.0),    # folded split, Q = 768, zero pad
+    ((20, 12, 700), (32, 16, 1152), 3, 15, 0.3),    # z pairs of 1152-long columns: split, rows past 576 folded
+    ((10, 8, 900), (32, 16, 1536), 100, 0, 0.0),    # z pairs of 1536-long columns, zero pad
 ])
 def test_xlast_matches_oracle_and_xfirst(vfd, orc, monkeypatch, shape, box, pad, epi, base):
     import torch
