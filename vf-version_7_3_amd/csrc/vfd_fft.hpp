@@ -117,16 +117,12 @@ constexpr Rot best_rot(double c, double s, double target = 1.0) {
         }
     return best;
 }
-constexpr double mag2(Rot r) { return (double)r.c * r.c + (double)r.s * r.s; }
-// W16^1 (both directions) and the radix-3 pair (-1/2, sqrt(3)/2)
+// W16^1 and the radix-3 pair (-1/2, sqrt(3)/2), both directions
 constexpr Rot kW16 = best_rot(0.92387953251128675613, 0.38268343236508977173);
 constexpr Rot kW3 = best_rot(-0.5, 0.86602540378443864676);
-// W8^1 = (1 - i)/sqrt(2): near 45 degrees the FP32 lattice moves |w|^2 in
-// steps of about 8e-8, so the forward pair stays the plain rounding and the
-// inverse pair is picked for |w_fwd|^2 |w_inv|^2 closest to 1 (a round trip
-// of 1 + 1.6e-8 instead of 1 - 6.8e-8)
-constexpr Rot kW8F = Rot{(float)0.70710678118654752440, (float)0.70710678118654752440};
-constexpr Rot kW8I = best_rot(0.70710678118654752440, 0.70710678118654752440, 1.0 / mag2(kW8F));
+// W8^1 = (1 - i)/sqrt(2): the plain rounding (near 45 degrees the FP32 lattice
+// moves |w|^2 in steps of about 8e-8)
+constexpr Rot kW8 = Rot{(float)0.70710678118654752440, (float)0.70710678118654752440};
 }  // namespace ce
 
 // ------------------------------------------------------------ butterflies
@@ -153,17 +149,17 @@ template <bool INV, typename T> struct Bfly<4, INV, T> {
 };
 
 // The rotation pairs (c, s) of the butterflies: FP32 takes the magnitude-picked
-// pairs of ce (forward and inverse may differ so that their magnitudes cancel
-// over a round trip), FP64 the correctly rounded values.
+// pairs of ce, the same in both directions, FP64 the correctly rounded values.
+// (Inverse pairs picked so that their magnitudes cancel the forward ones over a
+// round trip gave each FP32 inverse pass a gain of its own, and the drift from
+// FP64 over many ticks grew: 100-tick rel-L2 9.8e-6 against 6.9e-6.)
 template <typename T, bool INV> struct Rots {
     static constexpr T w8c = T(0.70710678118654752440), w8s = T(0.70710678118654752440);
     static constexpr T w16c = T(0.92387953251128675613), w16s = T(0.38268343236508977173);
     static constexpr T w3c = T(-0.5), w3s = T(0.86602540378443864676);
 };
 template <bool INV> struct Rots<float, INV> {
-    static constexpr ce::Rot w8 = INV ? ce::kW8I : ce::kW8F;
-    static constexpr ce::Rot w16 = INV ? ce::best_rot(ce::kW16.c, ce::kW16.s, 1.0 / ce::mag2(ce::kW16)) : ce::kW16;
-    static constexpr ce::Rot w3 = INV ? ce::best_rot(ce::kW3.c, ce::kW3.s, 1.0 / ce::mag2(ce::kW3)) : ce::kW3;
+    static constexpr ce::Rot w8 = ce::kW8, w16 = ce::kW16, w3 = ce::kW3;
     static constexpr float w8c = w8.c, w8s = w8.s, w16c = w16.c, w16s = w16.s, w3c = w3.c, w3s = w3.s;
 };
 
@@ -253,15 +249,14 @@ template <bool INV, typename T> struct Bfly<3, INV, T> {
 };
 
 // cos and sin of 2 pi q / R, q < R, as the rotation pairs of Rots (FP32:
-// magnitude-picked, the inverse pair compensating the forward one)
+// magnitude-picked, the same in both directions)
 template <typename T, bool INV, int R> struct OddTab {
     T c[R], s[R];
     constexpr OddTab() : c(), s() {
         for (int q = 0; q < R; ++q) {
             const double a = 2.0 * ce::PI * double(q) / double(R);
             if constexpr (sizeof(T) == 4) {
-                const ce::Rot f = ce::best_rot(ce::cos_(a), ce::sin_(a));
-                const ce::Rot r = INV ? ce::best_rot(ce::cos_(a), ce::sin_(a), 1.0 / ce::mag2(f)) : f;
+                const ce::Rot r = ce::best_rot(ce::cos_(a), ce::sin_(a));
                 c[q] = r.c;
                 s[q] = r.s;
             } else {
