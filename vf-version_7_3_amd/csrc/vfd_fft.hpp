@@ -163,16 +163,39 @@ template <bool INV> struct Rots<float, INV> {
     static constexpr float w8c = w8.c, w8s = w8.s, w16c = w16.c, w16s = w16.s, w3c = w3.c, w3s = w3.s;
 };
 
+// FP32: 1/sqrt(2) as hi + lo.  The FP32 value alone is the only pair near 45
+// degrees and has |w|^2 - 1 = -3.4e-8, the same on every path through a W8
+// rotation; (x -+ y) hi + (x -+ y) lo carries no such gain.
+constexpr float kRsqrt2Hi = 0.70710678118654752440f;
+constexpr float kRsqrt2Lo = (float)(0.70710678118654752440 - (double)kRsqrt2Hi);
+__device__ __forceinline__ float rsqrt2_mul(float p) { return fmaf(p, kRsqrt2Hi, p * kRsqrt2Lo); }
+// the same for sin(2 pi/3) of the radix-3 butterfly (its best FP32 pair is
+// off by -1.1e-8 in |w|^2).  W16's pair is left single (+7e-9): carrying it
+// as hi + lo too made the step 0.5 % slower for a smaller rel-L2 (100-tick
+// precision run 3.4e-6 -> 2.6e-6) and a larger mass drift (2.0e-6 -> 2.5e-6).
+constexpr double kS3 = 0.86602540378443864676;
+constexpr float kS3Hi = (float)kS3, kS3Lo = (float)(kS3 - (double)(float)kS3);
+
 // multiply by W8^1 = (1 - i)/sqrt(2) (fwd) / (1 + i)/sqrt(2) (inv), as the
 // pair (a, b) ~ (1/sqrt(2), 1/sqrt(2)): c (a -+ i b)
 template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_1(cx<T> c) {
-    constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
-    return INV ? cx<T>{c.x * a - c.y * b, c.y * a + c.x * b} : cx<T>{c.x * a + c.y * b, c.y * a - c.x * b};
+    if constexpr (sizeof(T) == 4) {
+        return INV ? cx<T>{rsqrt2_mul(c.x - c.y), rsqrt2_mul(c.y + c.x)}
+                   : cx<T>{rsqrt2_mul(c.x + c.y), rsqrt2_mul(c.y - c.x)};
+    } else {
+        constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
+        return INV ? cx<T>{c.x * a - c.y * b, c.y * a + c.x * b} : cx<T>{c.x * a + c.y * b, c.y * a - c.x * b};
+    }
 }
 // multiply by W8^3 = (-1 - i)/sqrt(2) (fwd) / (-1 + i)/sqrt(2) (inv): c (-a -+ i b)
 template <bool INV, typename T> __device__ __forceinline__ cx<T> w8_3(cx<T> c) {
-    constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
-    return INV ? cx<T>{-c.x * a - c.y * b, c.x * b - c.y * a} : cx<T>{c.y * b - c.x * a, -c.y * a - c.x * b};
+    if constexpr (sizeof(T) == 4) {
+        return INV ? cx<T>{rsqrt2_mul(-c.x - c.y), rsqrt2_mul(c.x - c.y)}
+                   : cx<T>{rsqrt2_mul(c.y - c.x), rsqrt2_mul(-c.y - c.x)};
+    } else {
+        constexpr T a = Rots<T, INV>::w8c, b = Rots<T, INV>::w8s;
+        return INV ? cx<T>{-c.x * a - c.y * b, c.x * b - c.y * a} : cx<T>{c.y * b - c.x * a, -c.y * a - c.x * b};
+    }
 }
 
 template <bool INV, typename T> struct Bfly<8, INV, T> {
@@ -240,9 +263,17 @@ template <bool INV, typename T> struct Bfly<3, INV, T> {
         cx<T> s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
         cx<T> a0 = v[0];
         v[0] = cadd(a0, s);
-        cx<T> t = cx<T>{a0.x + c3 * s.x, a0.y + c3 * s.y};
+        cx<T> t, sd;
+        if constexpr (sizeof(T) == 4) {
+            // FP32: c3 = -1/2 exactly, s3 as hi + lo
+            t = cx<T>{fmaf(-0.5f, s.x, a0.x), fmaf(-0.5f, s.y, a0.y)};
+            sd = cx<T>{fmaf(d.x, kS3Hi, d.x * kS3Lo), fmaf(d.y, kS3Hi, d.y * kS3Lo)};
+        } else {
+            t = cx<T>{a0.x + c3 * s.x, a0.y + c3 * s.y};
+            sd = cscale(d, s3);
+        }
         // fwd: y1 = t - i s3 d ; y2 = t + i s3 d
-        cx<T> m = mul_mi<INV>(cscale(d, s3));
+        cx<T> m = mul_mi<INV>(sd);
         v[1] = cadd(t, m);
         v[2] = csub(t, m);
     }

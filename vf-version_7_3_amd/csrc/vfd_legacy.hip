@@ -642,7 +642,7 @@ int tick_check(const char* who, const c_ctx& cctx, const float* baseChem, bool d
 }
 
 // the host deltas are cleared once their copy has landed, while the device works
-std::vector<std::thread> clear_host_deltas(Pipe* pp, float* const* h_dchem, size_t bytes) {
+static std::vector<std::thread> clear_host_deltas(Pipe* pp, float* const* h_dchem, size_t bytes) {
     std::vector<std::thread> clear;
     for (int ic = 0; ic < N_CHEM; ++ic)
         clear.emplace_back([pp, ic, h_dchem, bytes] {
