@@ -91,8 +91,11 @@ struct XlDesc {
 
 // Largest window radius the direct z pass takes (separable spectrum, FP32);
 // the separable table carries the taps g_0 .. g_kZDirectMaxR.  Wider windows
-// (20, 24 taps a side) compiled to scratch-memory window accesses and were
-// slower than the FFT form, which they keep.
+// keep the FFT form: with a loop-rolled window, radius 20 and 24 compiled to
+// scratch-memory window accesses; fully unrolled (every window index a
+// constant), radius 20 held its window in 194 VGPRs without scratch but, at two
+// waves per SIMD, ran slower than the FFT form (0.60 against 0.53 ms per field
+// on 1024x512x128), and capped at three or four waves it spilled.
 constexpr int kZDirectMaxR = 16;
 
 // Every per-field array is addressed with 32-bit byte offsets.
