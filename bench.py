@@ -1035,7 +1035,7 @@ def main():
             "mix_gbs": mix_gbs,
             "kernels_frac_of_mix": {n: (g / mix_gbs[LAUNCH_MIX[n]] if g > 0 else None)
                                     for (n, _), g in zip(launches, kern_gbs)},
-            # the survey's byte model with a streamed real K^ (8N + 32 A1 + 32 A2 + 4 A3):
+            # the five-pass byte model with a streamed real K^ (8N + 32 A1 + 32 A2 + 4 A3):
             # counts 4 A3 per field that the separable spectrum never reads
             "model_bytes_per_step": model_bytes,
             "model_frac": model_bytes / (mean_step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
